@@ -1,0 +1,225 @@
+"""bench.py -- FP32 GEMM M=N=K=8192 on one MI355X through numc_matmul_ex.
+
+Workload (the headline metric's configuration): A, B ~ N(0,1) float32 from the
+documented seeded generator, resident in HBM before timing.  One step = one
+complete GEMM call (pack kernels + MFMA kernel) of the split-BF16 method;
+the plain FP32 and the per-row/column-scaled FP8 methods are timed the same
+way and reported beside it.  value = 2*M*N*K / ms_per_step (FP32-equivalent
+TFLOP/s; extra MFMA products are not counted as work).
+
+Errors: max componentwise |C - R| / (|A||B|) over all 67M entries, with R and
+|A||B| from the library's own FP64 MFMA GEMM (validated against the CPU oracle
+by the test suite), for the N(0,1) inputs and an ill-scaled set (rows of A and
+columns of B scaled by 2^u, u integer uniform in [-20, 20]).
+
+roofline: the MFMA kernel of the split-BF16 method; its average duration comes
+from HIP events the library records on its own stream around each launch.
+cpu_baseline: numc's own numc_matmul (oracle/_ref, built from numc's sources,
+AVX2 + OpenMP) on the same bytes, min of 2 after a warmup, on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GEMM TFLOPS/%peak: split-BF16 vs plain FP32 at ≤2× FP32 err vs FP64 oracle"
+PEAK = {"f32": 157.3, "split_bf16": 2516.0 / 3.0, "fp8": 5030.0, "f64": 78.6}   # TFLOP/s, MI355X dense
+HBM_PEAK = 8000.0  # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_method(nc, ctx, A, B, C, method, family, steps, warmup):
+    for _ in range(warmup):
+        nc.check(nc.matmul(A, B, C, method), "warmup")
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(1, family)
+    ctx.profile_read(0, family)
+    ctx.mark(0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nc.check(nc.matmul(A, B, C, method), "step")
+    ctx.mark(1)
+    ctx.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3
+    region = ctx.elapsed_ms(0, 1)
+    gemm_ms, gemm_n = ctx.profile_read(1, family)
+    pack_ms, pack_n = ctx.profile_read(0, family)
+    ctx.profile(False)
+    return {"ms_per_step": region / steps, "wall_ms_per_step": wall / steps,
+            "kernel_ms": gemm_ms / max(gemm_n, 1), "kernel_launches": gemm_n,
+            "pack_ms": pack_ms / max(pack_n, 1)}
+
+
+def errors_vs_fp64(nc, ctx, a, b, outs, n):
+    """componentwise max error of each output against the FP64 MFMA reference"""
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    res = {}
+    R = ctx.array((n, n), nc.FLOAT64)
+    S = ctx.array((n, n), nc.FLOAT64)
+    for x, y, dst in ((a64, b64, R), (np.abs(a64), np.abs(b64), S)):
+        X, Y = ctx.from_numpy(x), ctx.from_numpy(y)
+        nc.check(nc.matmul(X, Y, dst), "fp64 reference")
+    r, s = R.numpy(), S.numpy()
+    for name, c in outs.items():
+        d = np.abs(c.astype(np.float64) - r)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = np.where(s > 0, d / s, np.where(d > 0, np.inf, 0.0))
+        res[name] = {"componentwise": float(np.max(q)), "normwise": float(np.max(d) / np.max(np.abs(r)))}
+    return res
+
+
+def cpu_baseline(n, seed, cores):
+    """numc's own AVX2 numc_matmul on the same bytes (oracle/_ref), bounded sample."""
+    import oracle
+    if not os.path.exists(oracle.REF_BENCH):
+        return None, "oracle/_ref not built"
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([oracle.REF_BENCH, "8", str(n), str(n), str(n), "n", str(seed), "1", "2"],
+                       capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return None, "numc_ref_bench failed: %s" % r.stderr[-200:]
+    return json.loads(r.stdout.strip().splitlines()[-1]), None
+
+
+def wsum(x):
+    w = np.ascontiguousarray(x).view(np.uint32).ravel().astype(np.uint64)
+    return int(np.sum(w * np.arange(1, w.size + 1, dtype=np.uint64)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
+              "'How it would shard'); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numc_amd as nc
+    n = args.n
+    flops = 2.0 * n * n * n
+    ctx = nc.Ctx()
+    ctx.set_sync(False)
+    log("bench: FP32 GEMM %d^3, N(0,1) inputs, seed %d" % (n, args.seed))
+    A = ctx.array((n, n), nc.FLOAT32)
+    B = ctx.array((n, n), nc.FLOAT32)
+    A.fill_random(nc.DIST_NORMAL, args.seed, 0)
+    B.fill_random(nc.DIST_NORMAL, args.seed, 1)
+    a_host, b_host = A.numpy(), B.numpy()
+    A.to_device()
+    B.to_device()
+    outs = {}
+    C = {}
+    for name in ("split_bf16", "f32", "fp8"):
+        C[name] = ctx.array((n, n), nc.FLOAT32, zeros=True)
+        C[name].to_device()
+
+    variants = {}
+    plan = [("split_bf16", nc.METHOD_SPLIT_BF16, nc.FAMILY_SPLIT_BF16),
+            ("f32", nc.METHOD_PLAIN, nc.FAMILY_F32),
+            ("fp8", nc.METHOD_FP8_SCALED, nc.FAMILY_FP8)]
+    for name, method, fam in plan:
+        log("bench: timing %s (%d warmup, %d steps)" % (name, args.warmup, args.steps))
+        t = time_method(nc, ctx, A, B, C[name], method, fam, args.steps, args.warmup)
+        t["tflops"] = flops / t["ms_per_step"] * 1e-9
+        t["kernel_tflops"] = flops / t["kernel_ms"] * 1e-9
+        t["peak_tflops"] = PEAK[name]
+        t["frac_of_peak"] = t["kernel_tflops"] / PEAK[name]
+        variants[name] = t
+        log("bench: %s %.3f ms/step, %.1f TFLOP/s (kernel %.3f ms, %.1f TFLOP/s)" %
+            (name, t["ms_per_step"], t["tflops"], t["kernel_ms"], t["kernel_tflops"]))
+
+    errs = None
+    if not args.no_errors:
+        log("bench: errors against the FP64 reference (N(0,1) inputs)")
+        for name in C:
+            outs[name] = C[name].numpy()
+        errs = {"normal": errors_vs_fp64(nc, ctx, a_host, b_host, outs, n)}
+        rng = np.random.default_rng(args.seed)
+        u = rng.integers(-20, 20, n, endpoint=True)
+        v = rng.integers(-20, 20, n, endpoint=True)
+        ai = np.ldexp(a_host, u[:, None]).astype(np.float32)
+        bi = np.ldexp(b_host, v[None, :]).astype(np.float32)
+        AI, BI = ctx.from_numpy(ai), ctx.from_numpy(bi)
+        outs_i = {}
+        for name, method, fam in plan:
+            O = ctx.array((n, n), nc.FLOAT32, zeros=True)
+            nc.check(nc.matmul(AI, BI, O, method), "ill-scaled")
+            outs_i[name] = O.numpy()
+        log("bench: errors against the FP64 reference (ill-scaled inputs)")
+        errs["ill_scaled"] = errors_vs_fp64(nc, ctx, ai, bi, outs_i, n)
+        for name in variants:
+            variants[name]["err_componentwise"] = errs["normal"][name]["componentwise"]
+            variants[name]["err_componentwise_ill_scaled"] = errs["ill_scaled"][name]["componentwise"]
+
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        log("bench: cpu baseline (numc AVX2 numc_matmul, %d threads)" % cores)
+        res, why = cpu_baseline(n, args.seed, cores)
+        if res is not None:
+            same = res["a_wsum"] == str(wsum(a_host)) and res["b_wsum"] == str(wsum(b_host))
+            cpu = {"value": res["gflops"] * 1e-3, "unit": "TFLOP/s", "cores": res["threads"],
+                   "kind": "reference",
+                   "sample": "numc_matmul float32 %d^3 (numc AVX2+OpenMP build), min of %d after %d warmup, "
+                             "same input bytes: %s" % (n, res["iters"], res["warmup"], same),
+                   "ms": res["ms_min"]}
+        else:
+            cpu = {"value": None, "unit": "TFLOP/s", "cores": cores, "kind": "reference", "sample": why}
+
+    s = variants["split_bf16"]
+    p = variants["f32"]
+    out = {
+        "metric": METRIC,
+        "value": round(s["tflops"], 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "FP32 GEMM M=N=K=%d, A,B ~ N(0,1) (seeded), split-BF16 method "
+                               "(hi*hi + hi*lo + lo*hi bf16 MFMA, FP32 accumulate), operands resident in HBM"
+                               % n, "M": n, "N": n, "K": n, "method": "split_bf16"},
+        "roofline": {"bound": "mfma", "achieved": round(s["kernel_tflops"], 3), "peak": round(PEAK["split_bf16"], 1),
+                     "unit": "TFLOP/s", "frac": round(s["kernel_tflops"] / PEAK["split_bf16"], 4),
+                     "traffic": None,
+                     "kernel": "k_gemm<MfBF16,...,MODE_SPLIT3> (avg %.4f ms over %d launches)"
+                               % (s["kernel_ms"], s["kernel_launches"])},
+        "cpu_baseline": cpu,
+        "split_vs_plain_speedup": round(s["tflops"] / p["tflops"], 3),
+        "variants": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                     for k, v in variants.items()},
+    }
+    if errs:
+        out["err_ratio_split_vs_plain"] = round(errs["normal"]["split_bf16"]["componentwise"] /
+                                                errs["normal"]["f32"]["componentwise"], 4)
+        out["err_ratio_split_vs_plain_ill_scaled"] = round(
+            errs["ill_scaled"]["split_bf16"]["componentwise"] / errs["ill_scaled"]["f32"]["componentwise"], 4)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,111 @@
+/* numc_amd.h -- MI355X additions to the numc C ABI.
+ *
+ * Nothing here replaces a numc entry point; each declaration is additive.
+ * Where an addition has a counterpart in numc the comment names it. */
+#ifndef NUMC_AMD_H
+#define NUMC_AMD_H
+
+#include "numc/array.h"
+#include "numc/export.h"
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- float32 GEMM methods ------------------------------------------------ */
+typedef enum {
+  NUMC_METHOD_AUTO = 0,       /* = PLAIN for float32; the only choice for other dtypes */
+  NUMC_METHOD_PLAIN = 1,      /* FP32 MFMA, FP32 accumulate */
+  NUMC_METHOD_SPLIT_BF16 = 2, /* a = hi + lo (bf16 each), hi*hi + hi*lo + lo*hi, FP32 accumulate */
+  NUMC_METHOD_FP8_SCALED = 3, /* per-row (A) / per-column (B) power-of-two scales, OCP e4m3, FP32 accumulate */
+} NumcMethod;
+
+typedef struct {
+  int method;   /* NumcMethod */
+  int config;   /* kernel configuration index for the family, -1 = let the selector choose */
+  int flags;    /* NUMC_OPT_* */
+} NumcMatmulOpts;
+
+#define NUMC_OPT_WIDEN 1 /* allow int8/uint8 inputs with an int32/uint32 output */
+
+/* numc_matmul with a method and configuration (no counterpart in numc; the
+ * default options give exactly numc_matmul).  With NUMC_OPT_WIDEN, a and b
+ * may be INT8 (or UINT8) and out INT32 (or UINT32): the exact sum mod 2^32. */
+NUMC_API int numc_matmul_ex(const NumcArray *a, const NumcArray *b, NumcArray *out,
+                            const NumcMatmulOpts *opts);
+
+/* Many independent problems out[i] = a[i] @ b[i] (no counterpart in numc; it
+ * is the batched client ROADMAP.md names as numc_bmm).  All problems share
+ * one dtype (and method); shapes may differ.  Runs as one persistent kernel
+ * launch, replayed from a cached HIP graph when the same arrays repeat. */
+NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArray *const *a,
+                                 NumcArray *const *b, NumcArray *const *out,
+                                 const NumcMatmulOpts *opts);
+
+/* ---- residency -------------------------------------------------------------
+ * By default the host copy is the current one: every GPU call uploads its
+ * inputs and downloads its output, so pointers from numc_array_data behave
+ * as in numc.  numc_array_to_device makes the device copy current for the
+ * array's whole storage (views included); GEMMs then read and write HBM
+ * directly until numc_array_to_host or numc_array_data brings it back. */
+NUMC_API int numc_array_to_device(NumcArray *arr);
+NUMC_API int numc_array_to_host(NumcArray *arr);
+NUMC_API int numc_array_on_device(const NumcArray *arr);
+/* device address of element 0 of the view (allocates the block's twin) */
+NUMC_API void *numc_array_device_data(const NumcArray *arr);
+
+/* ---- execution control ----------------------------------------------------- */
+/* synchronous != 0 (default): calls return after the GPU finished.
+ * 0: calls return after enqueueing on the context's stream; results are
+ * ready after numc_ctx_synchronize (numc_array_data also waits). */
+NUMC_API int numc_ctx_set_sync(NumcCtx *ctx, int synchronous);
+NUMC_API int numc_ctx_synchronize(NumcCtx *ctx);
+NUMC_API void *numc_ctx_stream(NumcCtx *ctx); /* hipStream_t of the context */
+/* HIP events on the context stream, slots 0..15 */
+NUMC_API int numc_ctx_timer_mark(NumcCtx *ctx, int slot);
+NUMC_API int numc_ctx_timer_elapsed(NumcCtx *ctx, int slot0, int slot1, float *ms);
+
+/* Kernel timing: with profiling on, HIP events are recorded on the context
+ * stream around every pack (category 0), GEMM (1), split-K reduce (2) and
+ * batched (3) launch; read returns the summed milliseconds and launch count
+ * of one category and family (family NUMC_FAMILY_COUNT = other kernels). */
+NUMC_API int numc_ctx_profile(NumcCtx *ctx, int enable);
+NUMC_API int numc_ctx_profile_read(NumcCtx *ctx, int category, int family, double *total_ms,
+                                   long long *launches, int reset);
+
+/* ---- seeded inputs ----------------------------------------------------------
+ * Fills a contiguous array with the documented generator: element i comes from
+ * xoshiro256** block i>>16 seeded by splitmix64 from seed, stream and block.
+ * dist: 0 uniform [lo,hi), 1 normal(0,1) (Box-Muller), 2 integer uniform
+ * [lo,hi] by rejection.  Host-side; leaves the host copy current. */
+NUMC_API int numc_array_fill_random(NumcArray *arr, int dist, uint64_t seed,
+                                    uint64_t stream, double lo, double hi);
+
+/* ---- kernel configurations ------------------------------------------------ */
+typedef enum {
+  NUMC_FAMILY_F32 = 0,
+  NUMC_FAMILY_SPLIT_BF16 = 1,
+  NUMC_FAMILY_FP8 = 2,
+  NUMC_FAMILY_F64 = 3,
+  NUMC_FAMILY_I8 = 4, /* every integer dtype up to 32 bits (int8 digit planes) */
+  NUMC_FAMILY_COUNT = 5
+} NumcFamily;
+
+typedef struct {
+  int bm, bn, kc;     /* block tile rows, cols, k-chunk */
+  int waves_m, waves_n;
+  int split_k;        /* number of K slices (1 = none) */
+} NumcKernelConfig;
+
+NUMC_API int numc_kernel_config_count(int family);
+NUMC_API int numc_kernel_config_get(int family, int index, NumcKernelConfig *out);
+/* family a call with these arguments would run (or a negative error code) */
+NUMC_API int numc_matmul_family(const NumcArray *a, const NumcArray *b,
+                                const NumcArray *out, const NumcMatmulOpts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

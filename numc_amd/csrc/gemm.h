@@ -1,0 +1,51 @@
+// gemm.h -- host interface of the GEMM kernels (gemm.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "numc/numc.h"
+
+struct NumcCtx;
+
+// one GEMM on device-resident operands; strides in elements
+struct GemmProblem {
+  int family;            // NumcFamily
+  NumcDType in_dt, out_dt;
+  int M, N, K;
+  const void *A;
+  long long rsa, csa;
+  const void *B;
+  long long rsb, csb;
+  void *C;
+  long long rso, cso;
+};
+
+struct GemmArgs;
+typedef void (*GemmLaunchFn)(const GemmArgs &, dim3, hipStream_t);
+
+struct GemmConfig {
+  NumcKernelConfig cfg;
+  int T; // rows per MFMA fragment
+  GemmLaunchFn launch;
+};
+
+struct PackArgs {
+  const void *src;
+  long long rs, ks; // element (r, k) at src[r*rs + k*ks]
+  int R, K;
+  int nRT, nKF;     // padded row-fragment and k-fragment counts
+  unsigned char *dst;
+  long long plane_bytes;
+  const int *row_exp; // FP8
+  int *flags;         // integer digit planes: nonzero mask
+};
+
+const GemmConfig *numc_gemm_config(int family, int idx);
+int numc_gemm_config_count(int family);
+// device bytes needed in ctx->ws (and ctx->ws_aux) for a problem under a config
+size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *aux_bytes);
+int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, void *ws, void *aux);
+int numc_naive_launch(NumcCtx *ctx, NumcDType dt, const void *a, const void *b, void *c, int M, int K, int N);
+int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p);
+// heuristic / learned choice of config index for a family and shape
+int numc_select_config(int family, int M, int N, int K, int flags);

@@ -1,0 +1,164 @@
+// gemm_kernels.h -- device code shared by the GEMM translation units.
+//
+// Operand layout ("packed fragment order").  Every operand the MFMA kernels
+// read has been rewritten by a pack kernel into 1 KiB fragments: 64 lanes x
+// 16 bytes, exactly what one wave loads for one MFMA operand.  Lane l of a
+// fragment holds row (l mod T) and the E = 16/elem consecutive k values
+// starting at (l div T)*E, so a fragment covers T rows and KF = (64/T)*E k.
+// Fragments of one plane are stored [row-fragment rt][k-fragment kf], so the
+// k-run of one row panel is contiguous; rows are zero padded to the block
+// tile and k to the k-chunk, which removes every bound check from the inner
+// loop.  B is packed as B^T (its rows are the output columns), so A and B use
+// the same code.  Planes (bf16 hi/lo, int8 digits) follow each other.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+#define NUMC_FRAG_BYTES 1024
+
+// ---------------------------------------------------------------------------
+// MFMA flavours.  T: rows per fragment; KF: k per fragment; FR: fragments of
+// each operand consumed by one mma() call.
+// ---------------------------------------------------------------------------
+
+struct MfF32 { // v_mfma_f32_32x32x2_f32 x4 per fragment
+  static constexpr int T = 32, KF = 8, FR = 1, NACC = 16;
+  using Acc = f32x16;
+  using Raw = float;
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a[0].x), __uint_as_float(b[0].x), c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a[0].y), __uint_as_float(b[0].y), c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a[0].z), __uint_as_float(b[0].z), c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a[0].w), __uint_as_float(b[0].w), c, 0, 0, 0);
+  }
+  __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+  __device__ static inline int col(int lane) { return lane & 31; }
+};
+
+struct MfBF16 { // v_mfma_f32_32x32x16_bf16
+  static constexpr int T = 32, KF = 16, FR = 1, NACC = 16;
+  using Acc = f32x16;
+  using Raw = float;
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+    bf16x8 av = __builtin_bit_cast(bf16x8, a[0]);
+    bf16x8 bv = __builtin_bit_cast(bf16x8, b[0]);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+  __device__ static inline int col(int lane) { return lane & 31; }
+};
+
+struct MfI8 { // v_mfma_i32_32x32x32_i8
+  static constexpr int T = 32, KF = 32, FR = 1, NACC = 16;
+  using Acc = i32x16;
+  using Raw = int;
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+    i32x4 av = __builtin_bit_cast(i32x4, a[0]);
+    i32x4 bv = __builtin_bit_cast(i32x4, b[0]);
+    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+  __device__ static inline int col(int lane) { return lane & 31; }
+};
+
+struct MfFP8 { // v_mfma_scale_f32_32x32x64_f8f6f4, both operands OCP e4m3, unit scales
+  static constexpr int T = 32, KF = 32, FR = 2, NACC = 16;
+  using Acc = f32x16;
+  using Raw = float;
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+    i32x8 av = __builtin_shufflevector(__builtin_bit_cast(i32x4, a[0]), __builtin_bit_cast(i32x4, a[1]), 0, 1, 2, 3, 4, 5, 6, 7);
+    i32x8 bv = __builtin_shufflevector(__builtin_bit_cast(i32x4, b[0]), __builtin_bit_cast(i32x4, b[1]), 0, 1, 2, 3, 4, 5, 6, 7);
+    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, 127, 0, 127);
+  }
+  __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+  __device__ static inline int col(int lane) { return lane & 31; }
+};
+
+struct MfF64 { // v_mfma_f64_16x16x4_f64 x2 per fragment
+  static constexpr int T = 16, KF = 8, FR = 1, NACC = 4;
+  using Acc = f64x4;
+  using Raw = double;
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+    double a0 = __hiloint2double((int)a[0].y, (int)a[0].x), a1 = __hiloint2double((int)a[0].w, (int)a[0].z);
+    double b0 = __hiloint2double((int)b[0].y, (int)b[0].x), b1 = __hiloint2double((int)b[0].w, (int)b[0].z);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+  __device__ static inline int col(int lane) { return lane & 15; }
+};
+
+// ---------------------------------------------------------------------------
+// kernel arguments
+// ---------------------------------------------------------------------------
+
+enum GemmMode { MODE_SINGLE = 0, MODE_SPLIT3 = 1 };
+enum GemmEpi { EPI_F32 = 0, EPI_F64 = 1, EPI_FP8 = 2, EPI_INT = 3 };
+
+struct GemmArgs {
+  const unsigned char *A, *B; // packed operands
+  long long a_plane, b_plane; // bytes per plane
+  int nKF;                    // k-fragments per row panel (padded K / KF)
+  int nKC;                    // k-chunks (padded K / KC)
+  int M, N;
+  int nbm, nbn;               // block grid
+  int ksplit;                 // K slices (gridDim.y)
+  // epilogue
+  void *C;
+  long long rso, cso;         // output strides in elements
+  int out_bytes;              // integer output element size (1, 2, 4)
+  int shift_pass;             // integer digit pass s (adds (sum << 8s))
+  const int *plane_flags;     // integer: [0] nonzero-plane mask of A, [1] of B (null: use pa/pb)
+  const int *row_exp, *col_exp; // FP8 power-of-two scale exponents
+  void *partial;              // split-K partial sums [ksplit][M][N] (raw accumulator type)
+  int nprod;                  // MODE_SINGLE: products concatenated along K
+  int pa[4], pb[4];           // plane of A / B for each product
+};
+
+// Products of integer digit pass s: (i, s - i) for i = pa0 .. pa0 + n - 1,
+// restricted to planes holding a nonzero digit (masks from the pack kernels).
+__device__ inline int int_products(int s, int maskA, int maskB, int *pa0, int *pb0) {
+  int na = 32 - __clz(maskA), nb = 32 - __clz(maskB);
+  int lo = max(0, s - nb + 1), hi = min(s, na - 1);
+  *pa0 = lo;
+  *pb0 = s - lo;
+  return hi >= lo ? hi - lo + 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// epilogue helpers
+// ---------------------------------------------------------------------------
+
+template <int EPI, class Raw>
+__device__ inline void epi_store(const GemmArgs &g, int row, int col, Raw v) {
+  const long long off = (long long)row * g.rso + (long long)col * g.cso;
+  if constexpr (EPI == EPI_F32) {
+    ((float *)g.C)[off] = (float)v;
+  } else if constexpr (EPI == EPI_F64) {
+    ((double *)g.C)[off] = (double)v;
+  } else if constexpr (EPI == EPI_FP8) {
+    ((float *)g.C)[off] = ldexpf((float)v, -(g.row_exp[row] + g.col_exp[col]));
+  } else {
+    uint32_t x = (uint32_t)(int)v << (8 * g.shift_pass);
+    if (g.out_bytes == 1) {
+      uint8_t *p = (uint8_t *)g.C + off;
+      *p = (uint8_t)(g.shift_pass ? (uint32_t)*p + x : x);
+    } else if (g.out_bytes == 2) {
+      uint16_t *p = (uint16_t *)g.C + off;
+      *p = (uint16_t)(g.shift_pass ? (uint32_t)*p + x : x);
+    } else {
+      uint32_t *p = (uint32_t *)g.C + off;
+      *p = g.shift_pass ? *p + x : x;
+    }
+  }
+}

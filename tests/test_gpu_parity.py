@@ -1,0 +1,309 @@
+"""GPU parity of numc_matmul / numc_matmul_naive / numc_matmul_ex against the
+oracle, through the C ABI.  Bars: bit-exact for every integer dtype; for
+floats the componentwise error |C - R| / (|A||B|) against the FP64 (or
+double-double) oracle stays under the bound stated in each test.
+
+The cases mirror numc's tests/matmul/test_matmul_naive.c and test_matmul.c
+(fixture tests/golden/numc_known_answers.json) and add random, strided,
+empty and full-range cases plus every kernel configuration of every family.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import numc_amd as nc
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "numc_known_answers.json")))
+NPT = {"int8": np.int8, "int16": np.int16, "int32": np.int32, "int64": np.int64, "uint8": np.uint8,
+       "uint16": np.uint16, "uint32": np.uint32, "uint64": np.uint64, "float32": np.float32,
+       "float64": np.float64}
+U = 2.0 ** -24   # binary32 unit roundoff
+U64 = 2.0 ** -53
+
+
+def gemm(ctx, a, b, out_dtype=None, method=0, config=-1, flags=0, naive=False):
+    A = ctx.from_numpy(a)
+    B = ctx.from_numpy(b)
+    O = ctx.array((a.shape[0], b.shape[1]), nc.DTYPE_OF_NP[np.dtype(out_dtype or a.dtype)], zeros=True)
+    if naive:
+        code = nc.lib().numc_matmul_naive(A.h, B.h, O.h)
+    else:
+        code = nc.matmul(A, B, O, method, config, flags)
+    nc.check(code, "matmul")
+    return O.numpy()
+
+
+def fill(shape, dt, expr):
+    i = np.arange(int(np.prod(shape)))
+    return eval(expr, {"i": i}).astype(dt).reshape(shape)
+
+
+def cw_err(c, a, b):
+    ref, absr = oracle.fref(a, b)
+    return oracle.err_metrics(c.astype(np.float64) if c.dtype == np.float64 else c, ref, absr)[0]
+
+
+# ---- numc's known answers ----------------------------------------------------
+
+@pytest.mark.parametrize("naive", [False, True])
+def test_closed_forms(ctx, naive):
+    for c in GOLD["closed_form"]:
+        for dt in c["dtypes"]:
+            got = gemm(ctx, np.array(c["a"], NPT[dt]), np.array(c["b"], NPT[dt]), naive=naive)
+            assert np.array_equal(got, np.array(c["expect"], NPT[dt])), (dt, naive)
+
+
+def test_identity(ctx):
+    for c in GOLD["identity"]:
+        dt, n = NPT[c["dtype"]], c["n"]
+        a = fill((n, n), dt, "i+1" if c["fill"] == "i+1" else "i%" + c["fill"].split("%")[1])
+        assert np.array_equal(gemm(ctx, a, np.eye(n, dtype=dt)), a), c["dtype"]
+
+
+def test_ones(ctx):
+    for c in GOLD["ones"]:
+        dt = NPT[c["dtype"]]
+        got = gemm(ctx, np.ones((c["M"], c["K"]), dt), np.ones((c["K"], c["N"]), dt))
+        assert np.all(got == c["K"])
+
+
+@pytest.mark.parametrize("idx", range(len(GOLD["cross"])))
+def test_cross_fills(ctx, idx):
+    c = GOLD["cross"][idx]
+    dt = NPT[c["dtype"]]
+    a = fill((c["M"], c["K"]), dt, c.get("fill_a", "i%7+1"))
+    b = fill((c["K"], c["N"]), dt, c.get("fill_b", "i%5+1"))
+    got = gemm(ctx, a, b)
+    naive = gemm(ctx, a, b, naive=True)
+    if np.issubdtype(dt, np.integer):
+        want = oracle.matmul_int(a, b)
+        assert np.array_equal(got, want) and np.array_equal(naive, want)
+    else:
+        ref, _ = oracle.fref(a, b)
+        assert np.max(np.abs(got - ref)) <= c["tol"]
+        assert np.max(np.abs(got - naive)) <= c["tol"]
+
+
+def test_int8_cross(ctx):
+    c = GOLD["int8_cross"]
+    a = fill((c["M"], c["K"]), np.int8, "i%2")
+    b = fill((c["K"], c["N"]), np.int8, "i%2")
+    assert np.array_equal(gemm(ctx, a, b), gemm(ctx, a, b, naive=True))
+    assert np.array_equal(gemm(ctx, a, b), oracle.matmul_int(a, b))
+
+
+@pytest.mark.parametrize("edge", GOLD["edges"], ids=lambda e: "%dx%dx%d-%s" % tuple(e[:4]))
+def test_edges(ctx, edge):
+    M, K, N, dtn, tol = edge
+    dt = NPT[dtn]
+    a = fill((M, K), dt, "i%7+1")
+    b = fill((K, N), dt, "i%5+1")
+    got = gemm(ctx, a, b)
+    if np.issubdtype(dt, np.integer):
+        assert np.array_equal(got, oracle.matmul_int(a, b))
+    else:
+        ref, _ = oracle.fref(a, b)
+        assert np.max(np.abs(got - ref)) <= tol
+
+
+# ---- every kernel configuration -----------------------------------------------
+
+SHAPES = [(300, 333, 200), (64, 64, 64), (1, 517, 3), (257, 31, 129)]
+
+
+def rand_f32(rng, shape):
+    return rng.uniform(-1, 1, shape).astype(np.float32)
+
+
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F32)))
+def test_f32_configs(ctx, cfg):
+    rng = np.random.default_rng(100 + cfg)
+    for M, K, N in SHAPES:
+        a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+        got = gemm(ctx, a, b, config=cfg)
+        # deterministic bound for K-term binary32 accumulation: gamma_K = K*u/(1-K*u)
+        assert cw_err(got, a, b) <= (K + 2) * U, (cfg, M, K, N)
+
+
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_SPLIT_BF16)))
+def test_split_bf16_configs(ctx, cfg):
+    rng = np.random.default_rng(200 + cfg)
+    for M, K, N in SHAPES:
+        a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+        got = gemm(ctx, a, b, method=nc.METHOD_SPLIT_BF16, config=cfg)
+        # per product: dropped lo*lo and the two 2^-17 representation residues
+        # (<= 3 * 2^-16 |a||b|), plus binary32 accumulation
+        assert cw_err(got, a, b) <= 3 * 2.0 ** -16 + (K + 2) * U, (cfg, M, K, N)
+
+
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_FP8)))
+def test_fp8_configs(ctx, cfg):
+    rng = np.random.default_rng(300 + cfg)
+    for M, K, N in SHAPES:
+        a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+        got = gemm(ctx, a, b, method=nc.METHOD_FP8_SCALED, config=cfg)
+        qa, qb = oracle.fp8_quantized_inputs(a, b)
+        ref, absr = oracle.fref(qa, qb)   # exact operands of the FP8 products
+        err = oracle.err_metrics(got, ref, absr)[0]
+        assert err <= (K + 2) * U, (cfg, M, K, N, err)
+
+
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F64)))
+def test_f64_configs(ctx, cfg):
+    rng = np.random.default_rng(400 + cfg)
+    for M, K, N in SHAPES:
+        a, b = rng.standard_normal((M, K)), rng.standard_normal((K, N))
+        got = gemm(ctx, a, b, config=cfg)
+        assert cw_err(got, a, b) <= (K + 2) * U64, (cfg, M, K, N)
+
+
+INTS = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32]
+
+
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_I8)))
+@pytest.mark.parametrize("dt", INTS, ids=lambda d: np.dtype(d).name)
+def test_int_configs_full_range(ctx, cfg, dt):
+    rng = np.random.default_rng(500 + cfg)
+    info = np.iinfo(dt)
+    for M, K, N in SHAPES:
+        a = rng.integers(info.min, info.max, (M, K), dtype=dt, endpoint=True)
+        b = rng.integers(info.min, info.max, (K, N), dtype=dt, endpoint=True)
+        assert np.array_equal(gemm(ctx, a, b, config=cfg), oracle.matmul_int(a, b)), (cfg, M, K, N)
+
+
+@pytest.mark.parametrize("dt", [np.int64, np.uint64], ids=lambda d: np.dtype(d).name)
+def test_int64(ctx, dt):
+    rng = np.random.default_rng(6)
+    info = np.iinfo(dt)
+    for M, K, N in [(33, 70, 45), (1, 1, 1), (64, 64, 64)]:
+        a = rng.integers(info.min, info.max, (M, K), dtype=dt, endpoint=True)
+        b = rng.integers(info.min, info.max, (K, N), dtype=dt, endpoint=True)
+        assert np.array_equal(gemm(ctx, a, b), oracle.matmul_int(a, b))
+        assert np.array_equal(gemm(ctx, a, b, naive=True), oracle.matmul_int(a, b))
+
+
+@pytest.mark.parametrize("dt", INTS + [np.float32, np.float64], ids=lambda d: np.dtype(d).name)
+def test_naive_random(ctx, dt):
+    rng = np.random.default_rng(8)
+    M, K, N = 45, 77, 39
+    if np.issubdtype(dt, np.integer):
+        info = np.iinfo(dt)
+        a = rng.integers(info.min, info.max, (M, K), dtype=dt, endpoint=True)
+        b = rng.integers(info.min, info.max, (K, N), dtype=dt, endpoint=True)
+        assert np.array_equal(gemm(ctx, a, b, naive=True), oracle.matmul_int(a, b))
+    else:
+        a, b = rng.standard_normal((M, K)).astype(dt), rng.standard_normal((K, N)).astype(dt)
+        u = U if dt == np.float32 else U64
+        assert cw_err(gemm(ctx, a, b, naive=True), a, b) <= (K + 2) * u
+
+
+def test_widen_int8_to_int32(ctx):
+    rng = np.random.default_rng(12)
+    for dt, odt in [(np.int8, np.int32), (np.uint8, np.uint32), (np.int8, np.uint32)]:
+        info = np.iinfo(dt)
+        a = rng.integers(info.min, info.max, (130, 1000), dtype=dt, endpoint=True)
+        b = rng.integers(info.min, info.max, (1000, 70), dtype=dt, endpoint=True)
+        got = gemm(ctx, a, b, out_dtype=odt, flags=nc.OPT_WIDEN)
+        assert np.array_equal(got, oracle.matmul_int(a, b, odt)), (dt, odt)
+
+
+# ---- views, residency, empty shapes -----------------------------------------------
+
+def test_strided_views(ctx):
+    rng = np.random.default_rng(13)
+    for dt in [np.float32, np.float64, np.int32, np.int8]:
+        if np.issubdtype(dt, np.integer):
+            base_a = rng.integers(-50, 50, (90, 70)).astype(dt)
+            base_b = rng.integers(-50, 50, (80, 60)).astype(dt)
+        else:
+            base_a = rng.standard_normal((90, 70)).astype(dt)
+            base_b = rng.standard_normal((80, 60)).astype(dt)
+        A = ctx.from_numpy(base_a).transpose()          # (70, 90), strides (es, 70es)
+        B = ctx.from_numpy(base_b).slice(0, 0, 80, 1)   # full view
+        B = B.slice(1, 1, 60, 2)                        # (80, 30) every other column
+        Bs = base_b[:, 1:60:2]
+        Asub = A.slice(1, 0, 80)                        # (70, 80)
+        O_full = ctx.from_numpy(np.full((70, 65), 7, dt))
+        O = O_full.slice(1, 2, 62, 2)                   # (70, 30) with gaps
+        nc.check(nc.matmul(Asub, B, O), "matmul strided")
+        want_a = base_a.T[:, :80]
+        full = O_full.numpy()
+        got = full[:, 2:62:2]
+        if np.issubdtype(dt, np.integer):
+            assert np.array_equal(got, oracle.matmul_int(np.ascontiguousarray(want_a), np.ascontiguousarray(Bs)))
+        else:
+            assert cw_err(got, np.ascontiguousarray(want_a), np.ascontiguousarray(Bs)) <= 100 * U
+        mask = np.ones(full.shape, bool)
+        mask[:, 2:62:2] = False
+        assert np.all(full[mask] == 7), "bytes between output columns must be left alone"
+
+
+def test_device_residency(ctx):
+    rng = np.random.default_rng(14)
+    a = rng.uniform(-1, 1, (200, 300)).astype(np.float32)
+    b = rng.uniform(-1, 1, (300, 100)).astype(np.float32)
+    A, B = ctx.from_numpy(a), ctx.from_numpy(b)
+    O = ctx.array((200, 100), nc.FLOAT32, zeros=True)
+    for x in (A, B, O):
+        x.to_device()
+        assert nc.lib().numc_array_on_device(x.h)
+    nc.check(nc.matmul(A, B, O), "matmul resident")
+    assert nc.lib().numc_array_on_device(O.h)
+    got = O.numpy()                      # copies back, host current again
+    assert not nc.lib().numc_array_on_device(O.h)
+    assert cw_err(got, a, b) <= 302 * U
+    # writing through the host pointer after numc_array_data is seen by the next call
+    B.write(np.zeros_like(b))
+    nc.check(nc.matmul(A, B, O), "matmul after write")
+    assert np.all(O.numpy() == 0)
+
+
+def test_async_mode(ctx):
+    ctx.set_sync(False)
+    a = np.ones((64, 64), np.float32)
+    A = ctx.from_numpy(a)
+    O = ctx.array((64, 64), nc.FLOAT32, zeros=True)
+    nc.check(nc.matmul(A, A, O), "async matmul")
+    assert np.all(O.numpy() == 64)       # numc_array_data waits for the stream
+
+
+def test_empty_shapes(ctx):
+    for dt in [np.float32, np.int32, np.float64, np.int8]:
+        z = gemm(ctx, np.zeros((5, 0), dt), np.zeros((0, 7), dt))
+        assert z.shape == (5, 7) and np.all(z == 0)
+        assert gemm(ctx, np.zeros((0, 4), dt), np.zeros((4, 3), dt)).shape == (0, 3)
+
+
+def test_config1_f32_512(ctx):
+    """f32 512^3, A, B uniform[-1,1) from the seeded generator, rel err <= 1e-5."""
+    A = ctx.array((512, 512), nc.FLOAT32)
+    B = ctx.array((512, 512), nc.FLOAT32)
+    A.fill_random(nc.DIST_UNIFORM, 1, 0, -1, 1)
+    B.fill_random(nc.DIST_UNIFORM, 1, 1, -1, 1)
+    O = ctx.array((512, 512), nc.FLOAT32, zeros=True)
+    nc.check(nc.lib().numc_matmul(A.h, B.h, O.h), "numc_matmul")
+    a, b = A.numpy(), B.numpy()
+    ref, absr = oracle.fref(a, b)
+    m = oracle.err_metrics(O.numpy(), ref, absr)
+    assert m[0] <= 1e-5 and m[1] <= 1e-5
+
+
+def test_batched_uneven(ctx):
+    rng = np.random.default_rng(15)
+    shapes = [(17, 33, 65), (200, 16, 40), (1, 1000, 2), (128, 128, 128), (90, 250, 31)]
+    As, Bs, Os, ref = [], [], [], []
+    for M, K, N in shapes:
+        a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+        As.append(ctx.from_numpy(a))
+        Bs.append(ctx.from_numpy(b))
+        Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+        ref.append((a, b))
+    for method in [nc.METHOD_PLAIN, nc.METHOD_SPLIT_BF16]:
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os, method), "batched")
+        for (a, b), O in zip(ref, Os):
+            assert cw_err(O.numpy(), a, b) <= 3 * 2.0 ** -16 + (a.shape[1] + 2) * U
