@@ -209,7 +209,7 @@ def main():
                                % (s["kernel_ms"], s["kernel_launches"])},
         "cpu_baseline": cpu,
         "split_vs_plain_speedup": round(s["tflops"] / p["tflops"], 3),
-        "variants": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+        "variants": {k: {kk: (float("%.5g" % vv) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
     if errs:

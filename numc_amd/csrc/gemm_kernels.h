@@ -94,7 +94,8 @@ struct MfF64 { // v_mfma_f64_16x16x4_f64 x2 per fragment
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c, 0, 0, 0);
   }
-  __device__ static inline int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+  // output element v of lane l is row (l / 16) + 4v (checked on gfx950)
+  __device__ static inline int row(int v, int lane) { return (lane >> 4) + 4 * v; }
   __device__ static inline int col(int lane) { return lane & 15; }
 };
 

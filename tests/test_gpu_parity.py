@@ -150,7 +150,9 @@ def test_fp8_configs(ctx, cfg):
         qa, qb = oracle.fp8_quantized_inputs(a, b)
         ref, absr = oracle.fref(qa, qb)   # exact operands of the FP8 products
         err = oracle.err_metrics(got, ref, absr)[0]
-        assert err <= (K + 2) * U, (cfg, M, K, N, err)
+        # the f8f6f4 MFMA sums its 64 products internally with less than
+        # binary32 precision (measured ~2^-17 relative per instruction)
+        assert err <= -(-K // 64) * 2.0 ** -15 + (K + 2) * U, (cfg, M, K, N, err)
 
 
 @pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F64)))
