@@ -143,6 +143,159 @@ __global__ void k_pack_fp8(PackArgs a) {
   *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// ---- tiled packs for f32 rows with unit k-stride ------------------------
+// A block stages a 32-row x 128-k tile through LDS: reads are whole 512-byte
+// row segments (coalesced), writes are whole 1 KiB fragments.  MODE_PK: 0
+// plain f32 (KF 8), 1 bf16 hi/lo (KF 16), 2 e4m3 with row exponents (KF 32).
+template <int MODE_PK>
+__global__ void __launch_bounds__(256) k_pack_f32_tiled(PackArgs a) {
+  constexpr int TK = 128, PITCH = TK + 4;
+  __shared__ float tile[32 * PITCH];
+  const int nkt = (a.nKF * (MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32) + TK - 1) / TK;
+  const long long rt = blockIdx.x / nkt;
+  const int kt = blockIdx.x % nkt;
+  const int tid = threadIdx.x;
+  const long long row0 = rt * 32;
+  const int k0 = kt * TK;
+  // 32 rows x 128 floats = 1024 float4, 4 per thread
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    int idx = tid + it * 256;
+    int r = idx >> 5, c4 = (idx & 31) * 4;
+    long long row = row0 + r;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row < a.R) {
+      const float *src = (const float *)a.src + row * a.rs;
+      int k = k0 + c4;
+      if (k + 3 < a.K && ((((uintptr_t)(src + k)) & 15) == 0)) {
+        v = *(const float4 *)(src + k);
+      } else {
+        if (k < a.K) v.x = src[k];
+        if (k + 1 < a.K) v.y = src[k + 1];
+        if (k + 2 < a.K) v.z = src[k + 2];
+        if (k + 3 < a.K) v.w = src[k + 3];
+      }
+    }
+    float *t = tile + r * PITCH + c4;
+    t[0] = v.x; t[1] = v.y; t[2] = v.z; t[3] = v.w;
+  }
+  __syncthreads();
+  constexpr int KF = MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32;
+  constexpr int E = KF / 2;          // elements per lane
+  constexpr int FRAGS = TK / KF;     // fragments in the tile
+  constexpr int UNITS = FRAGS * 64;
+  for (int u = tid; u < UNITS; u += 256) {
+    const int f = u >> 6, lane = u & 63;
+    const int kf = kt * FRAGS + f;
+    if (kf >= a.nKF) continue;
+    const int r = lane & 31;
+    const float *t = tile + r * PITCH + f * KF + (lane >> 5) * E;
+    const long long off = (rt * a.nKF + kf) * 1024 + lane * 16;
+    if constexpr (MODE_PK == 0) {
+      *(uint4 *)(a.dst + off) = make_uint4(__float_as_uint(t[0]), __float_as_uint(t[1]),
+                                           __float_as_uint(t[2]), __float_as_uint(t[3]));
+    } else if constexpr (MODE_PK == 1) {
+      uint32_t hi[4], lo[4];
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        float x0 = t[e], x1 = t[e + 1];
+        uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
+        uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
+        uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
+        hi[e / 2] = (h0 & 0xFFFFu) | (h1 << 16);
+        lo[e / 2] = (l0 & 0xFFFFu) | (l1 << 16);
+      }
+      *(uint4 *)(a.dst + off) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+      *(uint4 *)(a.dst + a.plane_bytes + off) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    } else {
+      const long long row = row0 + r;
+      const int ex = row < a.R ? a.row_exp[row] : 0;
+      uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int e = 0; e < 16; e++) w[e >> 2] |= e4m3_rne(ldexpf(t[e], ex)) << (8 * (e & 3));
+      *(uint4 *)(a.dst + off) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
+// Same for sources whose rows are contiguous (unit row stride, e.g. B^T of a
+// row-major B): a 128-row x 32-k tile, read as 512-byte runs along rows and
+// transposed through LDS.
+template <int MODE_PK>
+__global__ void __launch_bounds__(256) k_pack_f32_tiled_rows(PackArgs a) {
+  constexpr int TR = 128, TK = 32, PITCH = TK + 1;
+  __shared__ float tile[TR * PITCH];
+  constexpr int KF = MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32;
+  const int nkt = (a.nKF * KF + TK - 1) / TK;
+  const int nrt4 = (a.nRT + 3) / 4;
+  const long long rb = blockIdx.x / nkt;   // block of 4 row fragments
+  const int kt = blockIdx.x % nkt;
+  const int tid = threadIdx.x;
+  const long long row0 = rb * TR;
+  const int k0 = kt * TK;
+  (void)nrt4;
+  // 32 k x 128 rows = 1024 float4 along rows, 4 per thread
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    int idx = tid + it * 256;
+    int kk = idx >> 5, r4 = (idx & 31) * 4;
+    int k = k0 + kk;
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (k < a.K) {
+      const float *src = (const float *)a.src + (long long)k * a.ks + row0 + r4;
+      long long row = row0 + r4;
+      if (row + 3 < a.R && ((((uintptr_t)src) & 15) == 0)) {
+        float4 q = *(const float4 *)src;
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (row + e < a.R) v[e] = src[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) tile[(r4 + e) * PITCH + kk] = v[e];
+  }
+  __syncthreads();
+  constexpr int E = KF / 2;
+  constexpr int FK = TK / KF;            // fragments along k in the tile
+  constexpr int UNITS = 4 * FK * 64;
+  for (int u = tid; u < UNITS; u += 256) {
+    const int f = u >> 6, lane = u & 63;
+    const int rl = f / FK, fk = f % FK;   // row fragment in tile, k fragment
+    const long long rt = rb * 4 + rl;
+    const int kf = kt * FK + fk;
+    if (rt >= a.nRT || kf >= a.nKF) continue;
+    const int r = rl * 32 + (lane & 31);
+    const float *t = tile + r * PITCH + fk * KF + (lane >> 5) * E;
+    const long long off = (rt * a.nKF + kf) * 1024 + lane * 16;
+    if constexpr (MODE_PK == 0) {
+      *(uint4 *)(a.dst + off) = make_uint4(__float_as_uint(t[0]), __float_as_uint(t[1]),
+                                           __float_as_uint(t[2]), __float_as_uint(t[3]));
+    } else if constexpr (MODE_PK == 1) {
+      uint32_t hi[4], lo[4];
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        float x0 = t[e], x1 = t[e + 1];
+        uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
+        uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
+        uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
+        hi[e / 2] = (h0 & 0xFFFFu) | (h1 << 16);
+        lo[e / 2] = (l0 & 0xFFFFu) | (l1 << 16);
+      }
+      *(uint4 *)(a.dst + off) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+      *(uint4 *)(a.dst + a.plane_bytes + off) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    } else {
+      const long long row = row0 + r;
+      const int ex = row < a.R ? a.row_exp[row] : 0;
+      uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int e = 0; e < 16; e++) w[e >> 2] |= e4m3_rne(ldexpf(t[e], ex)) << (8 * (e & 3));
+      *(uint4 *)(a.dst + off) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
 // f64 -> f64 (T=16, 2 per lane, KF=8)
 __global__ void k_pack_f64(PackArgs a) {
   long long t = blockIdx.x * 256LL + threadIdx.x;
@@ -244,7 +397,24 @@ __global__ void k_exps(const uint32_t *bits, int *ex, int n) {
 // products concatenated along K (integer digit passes).
 // ===========================================================================
 
-template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI>
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void global_void_t;
+
+// s_waitcnt vmcnt(n) that leaves the LDS/export counters alone (gfx9 encoding:
+// vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] at [15:14])
+#define WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
+
+template <int N> __device__ inline void wait_vmcnt_upto(int n) {
+  // n is wave-uniform and at most N; s_waitcnt needs an immediate
+  if constexpr (N >= 0) {
+    if (n >= N) { WAIT_VMCNT(N); return; }
+    wait_vmcnt_upto<N - 1>(n);
+  } else {
+    WAIT_VMCNT(0);
+  }
+}
+
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
 __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   const GemmArgs g = gin; // local copy: lambdas below capture it by reference
   constexpr int T = MF::T, KF = MF::KF, FR = MF::FR;
@@ -254,15 +424,19 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   constexpr int P = (MODE == MODE_SPLIT3) ? 2 : 1;
   constexpr int RA = BM / T, RB = BN / T;
   constexpr int A_FR = P * RA * KFC, B_FR = P * RB * KFC, FRC = A_FR + B_FR;
-  constexpr int LA = A_FR / NW, LB = B_FR / NW;
+  constexpr int LA = A_FR / NW, LB = B_FR / NW, LPW = LA + LB;
   static_assert(A_FR % NW == 0 && B_FR % NW == 0, "fragments must split evenly over waves");
   static_assert(KFC % FR == 0 && TM >= 1 && TN >= 1, "bad tile");
+  static_assert(NS >= 2 && LPW * (NS - 2) <= 63, "too many loads in flight for vmcnt");
   using Acc = typename MF::Acc;
   using Raw = typename MF::Raw;
 
-  __shared__ u32x4 lds[2 * FRC * 64];
+  // NS stages of FRC fragments, filled straight from HBM by global_load_lds
+  __shared__ u32x4 lds[NS * FRC * 64];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // the wave index is uniform: keep everything derived from it in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave / WGN, wn = wave % WGN;
 
   // block -> tile: blocks with equal pid % 8 share an XCD, so give each XCD
@@ -286,19 +460,21 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   const int c0 = (int)((long long)blockIdx.y * V / g.ksplit);
   const int c1 = (int)((long long)(blockIdx.y + 1) * V / g.ksplit);
 
-  long long offA[LA], offB[LB];
+  // wave-uniform byte offsets of the fragments this wave brings into LDS
+  long long uoffA[LA], uoffB[LB];
 #pragma unroll
   for (int i = 0; i < LA; i++) {
     int f = wave + i * NW;
     int p = f / (RA * KFC), rt = (f / KFC) % RA, kf = f % KFC;
-    offA[i] = p * g.a_plane + ((long long)(bm * RA + rt) * g.nKF + kf) * NUMC_FRAG_BYTES + lane * 16;
+    uoffA[i] = p * g.a_plane + ((long long)(bm * RA + rt) * g.nKF + kf) * NUMC_FRAG_BYTES;
   }
 #pragma unroll
   for (int i = 0; i < LB; i++) {
     int f = wave + i * NW;
     int p = f / (RB * KFC), rt = (f / KFC) % RB, kf = f % KFC;
-    offB[i] = p * g.b_plane + ((long long)(bn * RB + rt) * g.nKF + kf) * NUMC_FRAG_BYTES + lane * 16;
+    uoffB[i] = p * g.b_plane + ((long long)(bn * RB + rt) * g.nKF + kf) * NUMC_FRAG_BYTES;
   }
+  const int loff = lane * 16;
 
   Acc acc[TM][TN];
 #pragma unroll
@@ -308,73 +484,83 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
 #pragma unroll
       for (int v = 0; v < MF::NACC; v++) acc[i][j][v] = 0;
 
-  u32x4 ra[LA], rb[LB];
-  auto gload = [&](int c) __attribute__((always_inline)) {
-    int q = c / g.nKC, kc = c - q * g.nKC;
-    long long ko = (long long)kc * KFC * NUMC_FRAG_BYTES;
-    const unsigned char *pA = g.A + ko + (long long)(pa0 + q) * g.a_plane;
-    const unsigned char *pB = g.B + ko + (long long)(pb0 - q) * g.b_plane;
-#pragma unroll
-    for (int i = 0; i < LA; i++) ra[i] = *(const u32x4 *)(pA + offA[i]);
-#pragma unroll
-    for (int i = 0; i < LB; i++) rb[i] = *(const u32x4 *)(pB + offB[i]);
-  };
-  auto sstore = [&](int s) __attribute__((always_inline)) {
+  // chunk c = product q * nKC + k-chunk kc: one 1 KiB global->LDS copy per
+  // fragment, lane l's 16 bytes landing at slot + 16*l
+  auto issue = [&](int c, int s) __attribute__((always_inline)) {
+    const int q = c / g.nKC, kc = c - q * g.nKC;
+    const long long ko = (long long)kc * KFC * NUMC_FRAG_BYTES;
+    const unsigned char *pA = g.A + ko + (long long)(pa0 + q) * g.a_plane + loff;
+    const unsigned char *pB = g.B + ko + (long long)(pb0 - q) * g.b_plane + loff;
     u32x4 *base = lds + s * FRC * 64;
 #pragma unroll
-    for (int i = 0; i < LA; i++) base[(wave + i * NW) * 64 + lane] = ra[i];
+    for (int i = 0; i < LA; i++)
+      __builtin_amdgcn_global_load_lds((global_void_t *)(pA + uoffA[i]),
+                                       (lds_void_t *)(base + (wave + i * NW) * 64), 16, 0, 0);
 #pragma unroll
-    for (int i = 0; i < LB; i++) base[(A_FR + wave + i * NW) * 64 + lane] = rb[i];
+    for (int i = 0; i < LB; i++)
+      __builtin_amdgcn_global_load_lds((global_void_t *)(pB + uoffB[i]),
+                                       (lds_void_t *)(base + (A_FR + wave + i * NW) * 64), 16, 0, 0);
   };
-  auto compute = [&](int s) __attribute__((always_inline)) {
+  // fragments of one k step, double buffered in registers
+  u32x4 af[2][P][TM][FR], bfr[2][P][TN][FR];
+  auto fload = [&](int s, int kf, int buf) __attribute__((always_inline)) {
     const u32x4 *base = lds + s * FRC * 64;
 #pragma unroll
-    for (int kf = 0; kf < KFC; kf += FR) {
-      u32x4 af[P][TM][FR], bfr[P][TN][FR];
+    for (int p = 0; p < P; p++)
 #pragma unroll
-      for (int p = 0; p < P; p++)
+      for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int i = 0; i < TM; i++)
+        for (int r = 0; r < FR; r++)
+          af[buf][p][i][r] = base[((p * RA + wm * TM + i) * KFC + kf + r) * 64 + lane];
 #pragma unroll
-          for (int r = 0; r < FR; r++)
-            af[p][i][r] = base[((p * RA + wm * TM + i) * KFC + kf + r) * 64 + lane];
+    for (int p = 0; p < P; p++)
 #pragma unroll
-      for (int p = 0; p < P; p++)
+      for (int j = 0; j < TN; j++)
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+        for (int r = 0; r < FR; r++)
+          bfr[buf][p][j][r] = base[(A_FR + (p * RB + wn * TN + j) * KFC + kf + r) * 64 + lane];
+  };
+  auto mfmas = [&](int buf) __attribute__((always_inline)) {
+    if constexpr (MODE == MODE_SPLIT3) {
 #pragma unroll
-          for (int r = 0; r < FR; r++)
-            bfr[p][j][r] = base[(A_FR + (p * RB + wn * TN + j) * KFC + kf + r) * 64 + lane];
-      if constexpr (MODE == MODE_SPLIT3) {
+      for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int i = 0; i < TM; i++)
+        for (int j = 0; j < TN; j++) {
+          MF::mma(acc[i][j], af[buf][1][i], bfr[buf][0][j]);
+          MF::mma(acc[i][j], af[buf][0][i], bfr[buf][1][j]);
+          MF::mma(acc[i][j], af[buf][0][i], bfr[buf][0][j]);
+        }
+    } else {
 #pragma unroll
-          for (int j = 0; j < TN; j++) {
-            MF::mma(acc[i][j], af[1][i], bfr[0][j]);
-            MF::mma(acc[i][j], af[0][i], bfr[1][j]);
-            MF::mma(acc[i][j], af[0][i], bfr[0][j]);
-          }
-      } else {
+      for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int i = 0; i < TM; i++)
-#pragma unroll
-          for (int j = 0; j < TN; j++) MF::mma(acc[i][j], af[0][i], bfr[0][j]);
-      }
+        for (int j = 0; j < TN; j++) MF::mma(acc[i][j], af[buf][0][i], bfr[buf][0][j]);
     }
   };
 
-  if (c0 < c1) {
-    gload(c0);
-    sstore(0);
-    __syncthreads();
-  }
+  // prologue: NS-1 chunks in flight
+#pragma unroll
+  for (int u = 0; u < NS - 1; u++)
+    if (c0 + u < c1) issue(c0 + u, u);
+
+  int stage = 0; // stage holding chunk c
   for (int c = c0; c < c1; c++) {
-    const int s = (c - c0) & 1;
-    const bool more = c + 1 < c1;
-    if (more) gload(c + 1);
-    compute(s);
-    if (more) sstore(s ^ 1);
+    // this wave's copies of chunk c are done once at most the later chunks'
+    // copies are outstanding; the barrier makes every wave's copies visible
+    wait_vmcnt_upto<LPW * (NS - 2)>(LPW * min(NS - 2, c1 - 1 - c));
     __syncthreads();
+    // the stage before this one held chunk c-1, which every wave has finished
+    if (c + NS - 1 < c1) issue(c + NS - 1, stage == 0 ? NS - 1 : stage - 1);
+    fload(stage, 0, 0);
+#pragma unroll
+    for (int kf = 0; kf < KFC; kf += FR) {
+      const int buf = (kf / FR) & 1;
+      if (kf + FR < KFC) fload(stage, kf + FR, buf ^ 1);
+      if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
+      mfmas(buf);
+      if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+    }
+    stage = stage == NS - 1 ? 0 : stage + 1;
   }
 
   // epilogue
@@ -458,48 +644,54 @@ __global__ void k_gemm_i64(const T *a, const T *b, T *c, int M, int K, int N, lo
 // configuration tables
 // ===========================================================================
 
-template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI>
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
 static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI>), grid, dim3(WGM * WGN * 64), 0, s, g);
+  hipLaunchKernelGGL((k_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, PRIO>), grid, dim3(WGM * WGN * 64), 0, s, g);
 }
 
-#define CFG(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT) \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI>}
+#define CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, PRIO) \
+  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, PRIO>}
+#define CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS) CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, 0)
+#define CFG(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT) CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, 2)
 
+// (family, tile, k-chunk, waves, split-K, LDS stages); stages x chunk bytes <= 160 KiB
 static const GemmConfig kF32[] = {
-    CFG(MfF32, 256, 256, 16, 2, 2, MODE_SINGLE, EPI_F32, 1),
-    CFG(MfF32, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F32, 1),
-    CFG(MfF32, 256, 128, 16, 2, 2, MODE_SINGLE, EPI_F32, 1),
-    CFG(MfF32, 128, 256, 16, 2, 2, MODE_SINGLE, EPI_F32, 1),
-    CFG(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 1),
-    CFG(MfF32, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F32, 4),
-    CFG(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 8),
+    CFGS(MfF32, 256, 256, 32, 2, 4, MODE_SINGLE, EPI_F32, 1, 2),
+    CFGS(MfF32, 256, 256, 16, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfF32, 256, 128, 16, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfF32, 128, 256, 16, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfF32, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfF32, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F32, 4, 4),
+    CFGS(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 8, 4),
 };
 static const GemmConfig kSplit[] = {
-    CFG(MfBF16, 256, 256, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1),
-    CFG(MfBF16, 128, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1),
-    CFG(MfBF16, 256, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1),
-    CFG(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1),
-    CFG(MfBF16, 128, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 4),
-    CFG(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 8),
+    CFGS(MfBF16, 256, 256, 16, 4, 2, MODE_SPLIT3, EPI_F32, 1, 4),
+    CFGS(MfBF16, 256, 256, 32, 2, 4, MODE_SPLIT3, EPI_F32, 1, 2),
+    CFGS(MfBF16, 256, 256, 16, 2, 2, MODE_SPLIT3, EPI_F32, 1, 4),
+    CFGS(MfBF16, 256, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 3),
+    CFGS(MfBF16, 128, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 4),
+    CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 4),
+    CFGS(MfBF16, 128, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 4, 4),
+    CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 8, 4),
 };
 static const GemmConfig kFP8[] = {
-    CFG(MfFP8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1),
-    CFG(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1),
-    CFG(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1),
-    CFG(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 4),
+    CFGS(MfFP8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
+    CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
+    CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
+    CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 4, 4),
 };
 static const GemmConfig kF64[] = {
-    CFG(MfF64, 128, 128, 16, 2, 2, MODE_SINGLE, EPI_F64, 1),
-    CFG(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 1),
-    CFG(MfF64, 128, 64, 16, 2, 2, MODE_SINGLE, EPI_F64, 1),
-    CFG(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 8),
+    CFGS(MfF64, 128, 128, 16, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
+    CFGS(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
+    CFGS(MfF64, 128, 64, 16, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
+    CFGS(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 8, 4),
 };
 static const GemmConfig kI8[] = {
-    CFG(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1),
-    CFG(MfI8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1),
-    CFG(MfI8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_INT, 1),
-    CFG(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 4),
+    CFGS(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
+    CFGS(MfI8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
+    CFGS(MfI8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
+    CFGS(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 4, 4),
 };
 #undef CFG
 
@@ -578,6 +770,25 @@ static void launch_digits(int planes, PackArgs &pk, unsigned nb, hipStream_t st)
 
 static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, hipStream_t st) {
   unsigned nb = blocks_for((long long)pk.nRT * pk.nKF * 64);
+  if (pk.ks == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_FP8)) {
+    // rows with unit k-stride: stage 32 x 128 tiles through LDS
+    const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
+    const long long nkt = cdiv((long long)pk.nKF * kf, 128);
+    dim3 grid((unsigned)((long long)pk.nRT * nkt));
+    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_f32_tiled<0>), grid, dim3(256), 0, st, pk);
+    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_f32_tiled<1>), grid, dim3(256), 0, st, pk);
+    else hipLaunchKernelGGL((k_pack_f32_tiled<2>), grid, dim3(256), 0, st, pk);
+    return;
+  }
+  if (pk.rs == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_FP8)) {
+    // rows contiguous: stage 128 x 32 tiles through LDS
+    const long long nkt = cdiv((long long)pk.nKF * family_kf(family), 32);
+    dim3 grid((unsigned)(cdiv(pk.nRT, 4) * nkt));
+    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_f32_tiled_rows<0>), grid, dim3(256), 0, st, pk);
+    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_f32_tiled_rows<1>), grid, dim3(256), 0, st, pk);
+    else hipLaunchKernelGGL((k_pack_f32_tiled_rows<2>), grid, dim3(256), 0, st, pk);
+    return;
+  }
   switch (family) {
   case NUMC_FAMILY_F32: hipLaunchKernelGGL(k_pack_f32, dim3(nb), dim3(256), 0, st, pk); break;
   case NUMC_FAMILY_SPLIT_BF16: hipLaunchKernelGGL(k_pack_split_bf16, dim3(nb), dim3(256), 0, st, pk); break;
