@@ -92,6 +92,59 @@ def cpu_baseline(n, seed, cores):
     return json.loads(r.stdout.strip().splitlines()[-1]), None
 
 
+def bench_batched(nc, count, seed, reps):
+    """4096 problems, M, N, K log-uniform integers in [16, 1024], U[-1,1) values:
+    one persistent graph-replayed launch vs one numc_matmul_ex call per problem."""
+    rng = np.random.default_rng(seed)
+    dims = np.clip(np.round(np.exp(rng.uniform(np.log(16), np.log(1024), (count, 3)))), 16, 1024).astype(int)
+    ctx = nc.Ctx()
+    ctx.set_sync(False)
+    res = {"problems": count, "flop": float(np.sum(2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]))}
+    for label, dt, odt, method, flags in (("f32", nc.FLOAT32, nc.FLOAT32, nc.METHOD_PLAIN, 0),
+                                          ("split_bf16", nc.FLOAT32, nc.FLOAT32, nc.METHOD_SPLIT_BF16, 0),
+                                          ("int8_to_int32", nc.INT8, nc.INT32, nc.METHOD_AUTO, nc.OPT_WIDEN)):
+        As, Bs, Os = [], [], []
+        for i, (M, N, K) in enumerate(dims):
+            A = ctx.array((M, K), dt)
+            B = ctx.array((K, N), dt)
+            if dt == nc.INT8:
+                A.fill_random(nc.DIST_INT, seed, 2 * i, -128, 127)
+                B.fill_random(nc.DIST_INT, seed, 2 * i + 1, -128, 127)
+            else:
+                A.fill_random(nc.DIST_UNIFORM, seed, 2 * i, -1, 1)
+                B.fill_random(nc.DIST_UNIFORM, seed, 2 * i + 1, -1, 1)
+            O = ctx.array((M, N), odt, zeros=True)
+            for x in (A, B, O):
+                x.to_device()
+            As.append(A)
+            Bs.append(B)
+            Os.append(O)
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), "batched warmup")  # builds the graph
+        ctx.synchronize()
+        ctx.mark(2)
+        for _ in range(reps):
+            nc.check(nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), "batched")
+        ctx.mark(3)
+        t_batched = ctx.elapsed_ms(2, 3) / reps
+        # one call per problem, same stream, no graph
+        for i in range(count):
+            nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate warmup")
+        ctx.synchronize()
+        ctx.mark(4)
+        for i in range(count):
+            nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate")
+        ctx.mark(5)
+        t_sep = ctx.elapsed_ms(4, 5)
+        res[label] = {"ms_batched": round(t_batched, 4), "tflops_batched": round(res["flop"] / t_batched * 1e-9, 2),
+                      "ms_separate_launches": round(t_sep, 3),
+                      "tflops_separate": round(res["flop"] / t_sep * 1e-9, 2),
+                      "speedup": round(t_sep / t_batched, 2)}
+        log("bench: batched %s: %.3f ms (%.1f TFLOP/s) vs %d separate calls %.2f ms" %
+            (label, t_batched, res["flop"] / t_batched * 1e-9, count, t_sep))
+    ctx.close()
+    return res
+
+
 def wsum(x):
     w = np.ascontiguousarray(x).view(np.uint32).ravel().astype(np.uint64)
     return int(np.sum(w * np.arange(1, w.size + 1, dtype=np.uint64)))
@@ -106,6 +159,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
@@ -184,6 +239,11 @@ def main():
         else:
             cpu = {"value": None, "unit": "TFLOP/s", "cores": cores, "kind": "reference", "sample": why}
 
+    batched = None
+    if not args.no_batched:
+        log("bench: batched uneven GEMM, %d problems" % args.batch)
+        batched = bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2))
+
     s = variants["split_bf16"]
     p = variants["f32"]
     out = {
@@ -212,6 +272,8 @@ def main():
         "variants": {k: {kk: (float("%.5g" % vv) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
+    if batched:
+        out["batched"] = batched
     if errs:
         out["err_ratio_split_vs_plain"] = round(errs["normal"]["split_bf16"]["componentwise"] /
                                                 errs["normal"]["f32"]["componentwise"], 4)

@@ -44,6 +44,9 @@ NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArray *const *a
                                  NumcArray *const *b, NumcArray *const *out,
                                  const NumcMatmulOpts *opts);
 
+/* number of times the context replayed its cached batched graph */
+NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx);
+
 /* ---- residency -------------------------------------------------------------
  * By default the host copy is the current one: every GPU call uploads its
  * inputs and downloads its output, so pointers from numc_array_data behave

@@ -1,20 +1,284 @@
-// batched.hip -- numc_matmul_batched: many independent GEMMs of uneven shape.
+// batched.hip -- numc_matmul_batched: many independent GEMMs of uneven shape
+// in one persistent launch, replayed from a cached HIP graph.
+//
+// No numc counterpart: numc's ROADMAP.md lists numc_bmm as the batched GEMM
+// client; numc_dot's batched loop (src/reduction/dot.c:409-421) calls the
+// single-problem kernel per batch.  Here the host builds, per call:
+//   * a pack descriptor per operand (all packed by one launch),
+//   * a problem table (packed operand addresses, output view),
+//   * a work list of (problem, tile row, tile col, tile class) items ordered
+//     longest-first (tile class chosen per problem, see numc_batch_class),
+// and captures fill-counter -> batched pack -> persistent GEMM into a graph.
+// The graph, its tables and its workspace are cached on the context and
+// replayed while the same arrays (same addresses, shapes, strides) come back.
 #include "gemm.h"
 #include "internal.h"
 
-extern "C" void numc_batch_cache_free(NumcCtx *) {}
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
 
 int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts,
                     bool finish); // matmul.cpp
+int numc_batch_class(int family, int M, int N, int K); // selector.cpp
+
+namespace {
+struct BatchCache {
+  std::vector<uint64_t> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void *dev = nullptr;
+  size_t dev_bytes = 0;
+  long long replays = 0;
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (dev) (void)hipFree(dev);
+    exec = nullptr;
+    graph = nullptr;
+    dev = nullptr;
+    dev_bytes = 0;
+    key.clear();
+  }
+};
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+inline long long cdivll(long long a, long long b) { return (a + b - 1) / b; }
+
+int stage_in(NumcCtx *ctx, const NumcArray *x, const void **dptr) {
+  unsigned char *d = numc_dev_ptr(x);
+  if (!d) return numc_get_error().code ? numc_get_error().code : NUMC_ERR_INTERNAL;
+  if (x->st->residency == RES_HOST) {
+    size_t span = numc_view_span(x);
+    if (span) HIP_TRY(hipMemcpyAsync(d, x->data, span, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy H2D");
+  }
+  *dptr = d;
+  return 0;
+}
+} // namespace
+
+extern "C" void numc_batch_cache_free(NumcCtx *ctx) {
+  if (!ctx || !ctx->batch_cache) return;
+  BatchCache *c = (BatchCache *)ctx->batch_cache;
+  c->release();
+  delete c;
+  ctx->batch_cache = nullptr;
+}
+
+extern "C" NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx) {
+  return ctx && ctx->batch_cache ? ((BatchCache *)ctx->batch_cache)->replays : 0;
+}
 
 extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArray *const *a,
                                             NumcArray *const *b, NumcArray *const *out,
                                             const NumcMatmulOpts *opts) {
   if (!ctx || (count && (!a || !b || !out)))
     return NUMC_SET_ERROR(NUMC_ERR_NULL, "matmul_batched: NULL argument");
+  if (count == 0) return 0;
+  if (count > (1u << 24)) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched: too many problems");
+  int family = -1;
   for (size_t i = 0; i < count; i++) {
-    int err = numc_matmul_one(a[i], b[i], out[i], opts, false);
-    if (err) return err;
+    int f = numc_matmul_family(a[i], b[i], out[i], opts);
+    if (f < 0) return f;
+    if (out[i]->ctx != ctx) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched: problem %zu belongs to another context", i);
+    if (i && f != family) return NUMC_SET_ERROR(NUMC_ERR_TYPE, "matmul_batched: problems mix dtypes or methods");
+    family = f;
+  }
+  const NumcDType in_dt = a[0]->dtype, out_dt = out[0]->dtype;
+  for (size_t i = 1; i < count; i++)
+    if (out[i]->dtype != out_dt) return NUMC_SET_ERROR(NUMC_ERR_TYPE, "matmul_batched: output dtypes differ");
+  const bool batched_kernel =
+      family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
+      (family == NUMC_FAMILY_I8 && in_dt == NUMC_DTYPE_INT8 && (out_dt == NUMC_DTYPE_INT8 || out_dt == NUMC_DTYPE_INT32));
+  int err = numc_gpu_init(ctx);
+  if (err) return err;
+  if (!batched_kernel) { // other dtypes: one launch sequence per problem on the same stream
+    for (size_t i = 0; i < count; i++)
+      if ((err = numc_matmul_one(a[i], b[i], out[i], opts, false))) return err;
+    return numc_finish(ctx);
+  }
+
+  // ---- stage operands and describe the problems ----
+  const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
+  const int planes = family == NUMC_FAMILY_SPLIT_BF16 ? 2 : 1;
+  const size_t ein = a[0]->elem_size, eout = out[0]->elem_size;
+  struct Prob {
+    int M, N, K, cls, bm, bn, kc;
+    const void *A, *B;
+    void *C;
+    long long rsa, csa, rsb, csb, rso, cso;
+    long long pla, plb, offA, offB;
+    int nRTa, nRTb, nKF;
+  };
+  std::vector<Prob> P(count);
+  long long packed = 0;
+  for (size_t i = 0; i < count; i++) {
+    Prob &p = P[i];
+    p.M = (int)a[i]->shape[0];
+    p.K = (int)a[i]->shape[1];
+    p.N = (int)b[i]->shape[1];
+    p.rsa = (long long)(a[i]->strides[0] / ein);
+    p.csa = (long long)(a[i]->strides[1] / ein);
+    p.rsb = (long long)(b[i]->strides[0] / ein);
+    p.csb = (long long)(b[i]->strides[1] / ein);
+    p.rso = (long long)(out[i]->strides[0] / eout);
+    p.cso = (long long)(out[i]->strides[1] / eout);
+    p.A = p.B = nullptr;
+    p.C = nullptr;
+    p.cls = p.bm = p.bn = p.kc = 0;
+    if (p.M == 0 || p.N == 0) continue;
+    if ((err = stage_in(ctx, a[i], &p.A)) || (err = stage_in(ctx, b[i], &p.B))) return err;
+    unsigned char *dc = numc_dev_ptr(out[i]);
+    if (!dc) return NUMC_ERR_INTERNAL;
+    if (out[i]->st->residency == RES_HOST && numc_view_span(out[i]) > out[i]->size * eout)
+      HIP_TRY(hipMemcpyAsync(dc, out[i]->data, numc_view_span(out[i]), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    p.C = dc;
+    p.cls = numc_batch_class(family, p.M, p.N, p.K);
+    numc_batch_class_shape(family, p.cls, &p.bm, &p.bn, &p.kc);
+    const long long kp = cdivll(p.K > 0 ? p.K : 1, p.kc) * p.kc;
+    p.nKF = (int)(kp / kf);
+    p.nRTa = (int)(cdivll(p.M, p.bm) * p.bm / 32);
+    p.nRTb = (int)(cdivll(p.N, p.bn) * p.bn / 32);
+    p.pla = (long long)p.nRTa * p.nKF * 1024;
+    p.plb = (long long)p.nRTb * p.nKF * 1024;
+    p.offA = packed;
+    packed += p.pla * planes;
+    p.offB = packed;
+    packed += p.plb * planes;
+  }
+
+  // ---- tables: work items longest-first ----
+  std::vector<size_t> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return (long long)P[x].K * P[x].bm * P[x].bn > (long long)P[y].K * P[y].bm * P[y].bn;
+  });
+  std::vector<int4> items;
+  std::vector<PackArgs> descs;
+  std::vector<long long> prefix(1, 0);
+  std::vector<BatchProblem> probs(count);
+  for (size_t i = 0; i < count; i++) {
+    const Prob &p = P[i];
+    if (!p.C) continue;
+    PackArgs da{}, db{};
+    da.src = p.A; da.rs = p.rsa; da.ks = p.csa; da.R = p.M; da.K = p.K;
+    da.nRT = p.nRTa; da.nKF = p.nKF; da.plane_bytes = p.pla;
+    db.src = p.B; db.rs = p.csb; db.ks = p.rsb; db.R = p.N; db.K = p.K;
+    db.nRT = p.nRTb; db.nKF = p.nKF; db.plane_bytes = p.plb;
+    descs.push_back(da);
+    prefix.push_back(prefix.back() + (long long)p.nRTa * p.nKF * 64);
+    descs.push_back(db);
+    prefix.push_back(prefix.back() + (long long)p.nRTb * p.nKF * 64);
+  }
+  for (size_t oi = 0; oi < count; oi++) {
+    const size_t i = order[oi];
+    const Prob &p = P[i];
+    if (!p.C) continue;
+    const int nbm = (int)cdivll(p.M, p.bm), nbn = (int)cdivll(p.N, p.bn);
+    for (int x = 0; x < nbm; x++)
+      for (int y = 0; y < nbn; y++) items.push_back(make_int4((int)i, x, y, p.cls));
+  }
+  const int ndesc = (int)descs.size();
+  const int nitems = (int)items.size();
+
+  // ---- cache key: everything the graph bakes in ----
+  std::vector<uint64_t> key;
+  key.reserve(4 + count * 12);
+  key.push_back((uint64_t)family);
+  key.push_back((uint64_t)out_dt);
+  key.push_back(count);
+  for (size_t i = 0; i < count; i++) {
+    const Prob &p = P[i];
+    uint64_t v[] = {(uint64_t)(uintptr_t)p.A, (uint64_t)(uintptr_t)p.B, (uint64_t)(uintptr_t)p.C,
+                    (uint64_t)p.M, (uint64_t)p.N, (uint64_t)p.K, (uint64_t)p.rsa, (uint64_t)p.csa,
+                    (uint64_t)p.rsb, (uint64_t)p.csb, (uint64_t)p.rso, (uint64_t)p.cso};
+    key.insert(key.end(), v, v + 12);
+  }
+  if (!ctx->batch_cache) ctx->batch_cache = new BatchCache();
+  BatchCache *cache = (BatchCache *)ctx->batch_cache;
+
+  if (nitems > 0 && key != cache->key) {
+    // device layout: descs | prefix | probs | items | counter | packed operands
+    size_t o_desc = 0;
+    size_t o_pref = align_up(o_desc + descs.size() * sizeof(PackArgs), 256);
+    size_t o_prob = align_up(o_pref + prefix.size() * sizeof(long long), 256);
+    size_t o_item = align_up(o_prob + probs.size() * sizeof(BatchProblem), 256);
+    size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
+    size_t o_pack = align_up(o_cnt + 256, 256);
+    size_t total = o_pack + (size_t)packed + 256;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    cache->release();
+    HIP_TRY(hipMalloc(&cache->dev, total), "hipMalloc(batched workspace)");
+    cache->dev_bytes = total;
+    unsigned char *base = (unsigned char *)cache->dev;
+    // fill in device addresses now that the workspace exists
+    size_t di = 0;
+    for (size_t i = 0; i < count; i++) {
+      const Prob &p = P[i];
+      BatchProblem &bp = probs[i];
+      memset(&bp, 0, sizeof bp);
+      if (!p.C) continue;
+      descs[di].dst = base + o_pack + p.offA;
+      descs[di + 1].dst = base + o_pack + p.offB;
+      di += 2;
+      bp.A = base + o_pack + p.offA;
+      bp.B = base + o_pack + p.offB;
+      bp.a_plane = p.pla;
+      bp.b_plane = p.plb;
+      bp.nKF = p.nKF;
+      bp.nKC = (int)(cdivll(p.K > 0 ? p.K : 1, p.kc));
+      bp.M = p.M;
+      bp.N = p.N;
+      bp.C = p.C;
+      bp.rso = p.rso;
+      bp.cso = p.cso;
+      bp.out_bytes = (int)eout;
+    }
+    HIP_TRY(hipMemcpyAsync(base + o_desc, descs.data(), descs.size() * sizeof(PackArgs), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    HIP_TRY(hipMemcpyAsync(base + o_pref, prefix.data(), prefix.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    HIP_TRY(hipMemcpyAsync(base + o_prob, probs.data(), probs.size() * sizeof(BatchProblem), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    HIP_TRY(hipMemcpyAsync(base + o_item, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    const int grid = std::max(1, std::min(nitems, 2 * cus));
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    int lerr = numc_batch_launch(family, (const PackArgs *)(base + o_desc), (const long long *)(base + o_pref), ndesc,
+                                 prefix.back(), (const BatchProblem *)(base + o_prob), (const int4 *)(base + o_item),
+                                 nitems, (int *)(base + o_cnt), grid, ctx->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (lerr) {
+      if (graph) (void)hipGraphDestroy(graph);
+      cache->release();
+      return lerr;
+    }
+    if (e != hipSuccess) {
+      cache->release();
+      return numc_hip_fail(e, "hipStreamEndCapture");
+    }
+    cache->graph = graph;
+    e = hipGraphInstantiate(&cache->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      cache->release();
+      return numc_hip_fail(e, "hipGraphInstantiate");
+    }
+    cache->key = key;
+  }
+
+  if (nitems > 0) {
+    numc_prof_begin(ctx, PROF_BATCHED, family);
+    HIP_TRY(hipGraphLaunch(cache->exec, ctx->stream), "hipGraphLaunch");
+    numc_prof_end(ctx);
+    cache->replays++;
+  }
+  for (size_t i = 0; i < count; i++) {
+    const Prob &p = P[i];
+    if (!p.C || out[i]->st->residency != RES_HOST) continue;
+    size_t span = numc_view_span(out[i]);
+    if (span) HIP_TRY(hipMemcpyAsync(out[i]->data, p.C, span, hipMemcpyDeviceToHost, ctx->stream), "D2H");
   }
   return numc_finish(ctx);
 }

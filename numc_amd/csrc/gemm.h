@@ -40,6 +40,21 @@ struct PackArgs {
   int *flags;         // integer digit planes: nonzero mask
 };
 
+// one problem of the persistent batched kernel (packed operands in a workspace)
+struct BatchProblem {
+  const unsigned char *A, *B;
+  long long a_plane, b_plane;
+  int nKF, nKC, M, N;
+  void *C;
+  long long rso, cso;
+  int out_bytes;
+};
+
+int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc);
+int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
+                      const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
+                      hipStream_t st);
+
 const GemmConfig *numc_gemm_config(int family, int idx);
 int numc_gemm_config_count(int family);
 // device bytes needed in ctx->ws (and ctx->ws_aux) for a problem under a config

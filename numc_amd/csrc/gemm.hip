@@ -12,6 +12,9 @@
 #include <algorithm>
 #include <cstring>
 
+static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+static inline unsigned blocks_for(long long n) { return (unsigned)cdiv(n, 256); }
+
 // ===========================================================================
 // number formats (device)
 // ===========================================================================
@@ -78,10 +81,8 @@ __global__ void k_zero_strided(unsigned char *c, int M, int N, long long rso, lo
 
 template <class S> __device__ inline S ld(const void *p, long long i) { return ((const S *)p)[i]; }
 
-// f32 -> f32 (T=32, 4 per lane, KF=8)
-__global__ void k_pack_f32(PackArgs a) {
-  long long t = blockIdx.x * 256LL + threadIdx.x;
-  if (t >= (long long)a.nRT * a.nKF * 64) return;
+// f32 -> f32 (T=32, 4 per lane, KF=8): unit t of a plane
+__device__ inline void pack_unit_f32(const PackArgs &a, long long t) {
   int lane = (int)(t & 63);
   long long fr = t >> 6;
   int kf = (int)(fr % a.nKF);
@@ -97,9 +98,7 @@ __global__ void k_pack_f32(PackArgs a) {
 }
 
 // f32 -> bf16 hi (plane 0) and lo (plane 1)  (T=32, 8 per lane, KF=16)
-__global__ void k_pack_split_bf16(PackArgs a) {
-  long long t = blockIdx.x * 256LL + threadIdx.x;
-  if (t >= (long long)a.nRT * a.nKF * 64) return;
+__device__ inline void pack_unit_split(const PackArgs &a, long long t) {
   int lane = (int)(t & 63);
   long long fr = t >> 6;
   int kf = (int)(fr % a.nKF);
@@ -121,6 +120,52 @@ __global__ void k_pack_split_bf16(PackArgs a) {
   }
   *(uint4 *)(a.dst + t * 16) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
   *(uint4 *)(a.dst + a.plane_bytes + t * 16) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+}
+
+// int8 -> one int8 plane (T=32, 16 per lane, KF=32); exact for int8 sources
+__device__ inline void pack_unit_i8(const PackArgs &a, long long t) {
+  int lane = (int)(t & 63);
+  long long fr = t >> 6;
+  int kf = (int)(fr % a.nKF);
+  long long row = (fr / a.nKF) * 32 + (lane & 31);
+  int k0 = kf * 32 + (lane >> 5) * 16;
+  uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    int k = k0 + e;
+    uint32_t x = (row < a.R && k < a.K) ? (uint32_t)(uint8_t)ld<int8_t>(a.src, row * a.rs + (long long)k * a.ks) : 0u;
+    w[e >> 2] |= x << (8 * (e & 3));
+  }
+  *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ void k_pack_f32(PackArgs a) {
+  long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t < (long long)a.nRT * a.nKF * 64) pack_unit_f32(a, t);
+}
+
+__global__ void k_pack_split_bf16(PackArgs a) {
+  long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t < (long long)a.nRT * a.nKF * 64) pack_unit_split(a, t);
+}
+
+// many operands in one launch: unit t belongs to the descriptor whose unit
+// range [prefix[i], prefix[i+1]) holds it
+template <int FAM>
+__global__ void k_pack_batched(const PackArgs *descs, const long long *prefix, int ndesc) {
+  const long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t >= prefix[ndesc]) return;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const PackArgs a = descs[lo];
+  const long long u = t - prefix[lo];
+  if constexpr (FAM == NUMC_FAMILY_F32) pack_unit_f32(a, u);
+  else if constexpr (FAM == NUMC_FAMILY_SPLIT_BF16) pack_unit_split(a, u);
+  else pack_unit_i8(a, u);
 }
 
 // f32 * 2^row_exp[row] -> e4m3  (T=32, 16 per lane, KF=32)
@@ -414,9 +459,20 @@ template <int N> __device__ inline void wait_vmcnt_upto(int n) {
   }
 }
 
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int NS>
+struct TileShape {
+  static constexpr int T = MF::T, KF = MF::KF;
+  static constexpr int KFC = KC / KF;
+  static constexpr int P = (MODE == MODE_SPLIT3) ? 2 : 1;
+  static constexpr int FRC = P * (BM / T) * KFC + P * (BN / T) * KFC;
+  static constexpr int LDS_VEC = NS * FRC * 64; // u32x4 elements of LDS
+};
+
+// One BM x BN output tile of one problem: k-chunks [c0, c1) of slice `slice`
+// (the range is computed here), accumulated in registers and stored.
 template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
-__global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
-  const GemmArgs g = gin; // local copy: lambdas below capture it by reference
+__device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &g, const int bm, const int bn,
+                                                                const int slice, u32x4 *lds) {
   constexpr int T = MF::T, KF = MF::KF, FR = MF::FR;
   constexpr int NW = WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / T, TN = WN / T;
@@ -431,24 +487,10 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   using Acc = typename MF::Acc;
   using Raw = typename MF::Raw;
 
-  // NS stages of FRC fragments, filled straight from HBM by global_load_lds
-  __shared__ u32x4 lds[NS * FRC * 64];
-
   const int tid = threadIdx.x, lane = tid & 63;
   // the wave index is uniform: keep everything derived from it in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave / WGN, wn = wave % WGN;
-
-  // block -> tile: blocks with equal pid % 8 share an XCD, so give each XCD
-  // a contiguous range of tiles, then walk tiles in groups of 8 block rows
-  int pid = blockIdx.x;
-  const int nblk = g.nbm * g.nbn;
-  if ((nblk & 7) == 0) pid = (pid & 7) * (nblk >> 3) + (pid >> 3);
-  const int per_group = 8 * g.nbn;
-  const int first = (pid / per_group) * 8;
-  const int gsz = min(g.nbm - first, 8);
-  const int bm = first + (pid % per_group) % gsz;
-  const int bn = (pid % per_group) / gsz;
 
   // plane products: product q uses plane (pa0 + q) of A and (pb0 - q) of B
   int nprod = g.nprod, pa0 = g.pa[0], pb0 = g.pb[0];
@@ -457,8 +499,8 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
     if (nprod == 0 && g.shift_pass > 0) return; // this digit pass adds nothing
   }
   const int V = nprod * g.nKC;
-  const int c0 = (int)((long long)blockIdx.y * V / g.ksplit);
-  const int c1 = (int)((long long)(blockIdx.y + 1) * V / g.ksplit);
+  const int c0 = (int)((long long)slice * V / g.ksplit);
+  const int c1 = (int)((long long)(slice + 1) * V / g.ksplit);
 
   // wave-uniform byte offsets of the fragments this wave brings into LDS
   long long uoffA[LA], uoffB[LB];
@@ -574,11 +616,29 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
         const int row = row0 + i * T + MF::row(v, lane), col = col0 + j * T + MF::col(lane);
         if (row < g.M && col < g.N) {
           if (g.ksplit > 1)
-            ((Raw *)g.partial)[((long long)blockIdx.y * g.M + row) * g.N + col] = (Raw)acc[i][j][v];
+            ((Raw *)g.partial)[((long long)slice * g.M + row) * g.N + col] = (Raw)acc[i][j][v];
           else
             epi_store<EPI, Raw>(g, row, col, (Raw)acc[i][j][v]);
         }
       }
+}
+
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
+__global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
+  const GemmArgs g = gin; // local copy: the tile code captures it by reference
+  // NS stages of fragments, filled straight from HBM by global_load_lds
+  __shared__ u32x4 lds[TileShape<MF, BM, BN, KC, WGM, WGN, MODE, NS>::LDS_VEC];
+  // block -> tile: blocks with equal pid % 8 share an XCD, so give each XCD
+  // a contiguous range of tiles, then walk tiles in groups of 8 block rows
+  int pid = blockIdx.x;
+  const int nblk = g.nbm * g.nbn;
+  if ((nblk & 7) == 0) pid = (pid & 7) * (nblk >> 3) + (pid >> 3);
+  const int per_group = 8 * g.nbn;
+  const int first = (pid / per_group) * 8;
+  const int gsz = min(g.nbm - first, 8);
+  const int bm = first + (pid % per_group) % gsz;
+  const int bn = (pid % per_group) / gsz;
+  gemm_tile<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, PRIO>(g, bm, bn, blockIdx.y, lds);
 }
 
 // split-K: sum the slices in slice order, then the final epilogue
@@ -638,6 +698,88 @@ __global__ void k_gemm_i64(const T *a, const T *b, T *c, int M, int K, int N, lo
     __syncthreads();
   }
   if (row < M && col < N) c[row * rso + col * cso] = (T)acc;
+}
+
+// ===========================================================================
+// persistent batched GEMM: every (problem, tile) work item of many uneven
+// problems in one launch.  Blocks take items from an atomic counter; each item
+// names its problem and its tile class (two shapes compiled in), so small
+// problems get small tiles and the queue is ordered longest-first.
+// ===========================================================================
+
+template <class MF, int MODE, int EPI, int BM1, int BN1, int KC1, int NS1, int BM2, int BN2, int KC2, int NS2>
+__global__ void __launch_bounds__(256, 2) k_gemm_batched(const BatchProblem *probs, const int4 *items, int nitems,
+                                                          int *counter) {
+  using S1 = TileShape<MF, BM1, BN1, KC1, 2, 2, MODE, NS1>;
+  using S2 = TileShape<MF, BM2, BN2, KC2, 2, 2, MODE, NS2>;
+  constexpr int LDSV = S1::LDS_VEC > S2::LDS_VEC ? S1::LDS_VEC : S2::LDS_VEC;
+  __shared__ u32x4 lds[LDSV];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads(); // also: the previous tile's LDS reads are over
+    const int it = s_item;
+    __syncthreads(); // everyone read s_item before thread 0 overwrites it
+    if (it >= nitems) return;
+    const int4 w = items[it];
+    const BatchProblem pr = probs[w.x];
+    GemmArgs g{};
+    g.A = pr.A;
+    g.B = pr.B;
+    g.a_plane = pr.a_plane;
+    g.b_plane = pr.b_plane;
+    g.nKF = pr.nKF;
+    g.nKC = pr.nKC;
+    g.M = pr.M;
+    g.N = pr.N;
+    g.ksplit = 1;
+    g.C = pr.C;
+    g.rso = pr.rso;
+    g.cso = pr.cso;
+    g.out_bytes = pr.out_bytes;
+    g.nprod = 1;
+    if (w.w == 0)
+      gemm_tile<MF, BM1, BN1, KC1, 2, 2, MODE, EPI, NS1, 0>(g, w.y, w.z, 0, lds);
+    else
+      gemm_tile<MF, BM2, BN2, KC2, 2, 2, MODE, EPI, NS2, 0>(g, w.y, w.z, 0, lds);
+  }
+}
+
+// tile classes of the batched kernel per family: {BM, BN, KC}
+static const int kBatchClass[3][2][3] = {
+    {{128, 128, 32}, {64, 64, 32}},   // f32
+    {{128, 128, 32}, {64, 64, 32}},   // split-BF16
+    {{128, 128, 64}, {64, 64, 64}},   // int8 (one digit plane)
+};
+
+int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc) {
+  int f = family == NUMC_FAMILY_F32 ? 0 : family == NUMC_FAMILY_SPLIT_BF16 ? 1 : family == NUMC_FAMILY_I8 ? 2 : -1;
+  if (f < 0 || cls < 0 || cls > 1) return -1;
+  *bm = kBatchClass[f][cls][0];
+  *bn = kBatchClass[f][cls][1];
+  *kc = kBatchClass[f][cls][2];
+  return 0;
+}
+
+int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
+                      const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, 1LL, 0u);
+  unsigned nb = blocks_for(units);
+  if (nb) {
+    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_F32>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_SPLIT_BF16>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+    else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+  }
+  if (family == NUMC_FAMILY_F32)
+    hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  else if (family == NUMC_FAMILY_SPLIT_BF16)
+    hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  else
+    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 4, 64, 64, 64, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return numc_hip_fail(e, "batched launch");
+  return 0;
 }
 
 // ===========================================================================
@@ -731,8 +873,6 @@ extern "C" NUMC_API int numc_kernel_config_get(int family, int index, NumcKernel
 // host launch of one GEMM problem
 // ===========================================================================
 
-static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
-static inline unsigned blocks_for(long long n) { return (unsigned)cdiv(n, 256); }
 
 // packed bytes of one plane: rows padded to the block tile, k to the chunk
 static void plane_geometry(const GemmConfig &c, int R, int K, int kf_elems, int block_rows, int *nRT, int *nKF,

@@ -309,3 +309,66 @@ def test_batched_uneven(ctx):
         nc.check(nc.matmul_batched(ctx, As, Bs, Os, method), "batched")
         for (a, b), O in zip(ref, Os):
             assert cw_err(O.numpy(), a, b) <= 3 * 2.0 ** -16 + (a.shape[1] + 2) * U
+
+
+# ---- batched (persistent kernel, cached HIP graph) ---------------------------------
+
+def _logu(rng, lo, hi, n):
+    return np.clip(np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), n))), lo, hi).astype(int)
+
+
+def test_batched_many_uneven_and_graph_replay(ctx):
+    rng = np.random.default_rng(16)
+    n = 300
+    Ms, Ns, Ks = _logu(rng, 1, 700, n), _logu(rng, 1, 700, n), _logu(rng, 1, 700, n)
+    As, Bs, Os, ref = [], [], [], []
+    for M, N, K in zip(Ms, Ns, Ks):
+        a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+        As.append(ctx.from_numpy(a))
+        Bs.append(ctx.from_numpy(b))
+        Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+        ref.append((a, b))
+    before = nc.lib().numc_ctx_batch_graph_replays(ctx.h)
+    for rep in range(3):
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+        for (a, b), O in zip(ref, Os):
+            assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+        # eager work between replays, and new host data for problem 0
+        other = gemm(ctx, *ref[1])
+        assert cw_err(other, *ref[1]) <= (ref[1][0].shape[1] + 2) * U
+        a0 = rand_f32(rng, ref[0][0].shape)
+        As[0].write(a0)
+        ref[0] = (a0, ref[0][1])
+    assert nc.lib().numc_ctx_batch_graph_replays(ctx.h) - before == 3
+
+
+def test_batched_int8_widen_and_wrap(ctx):
+    rng = np.random.default_rng(17)
+    shapes = [(40, 300, 70), (128, 64, 128), (3, 5, 7), (260, 1000, 90)]
+    for odt, flags in [(np.int32, nc.OPT_WIDEN), (np.int8, 0)]:
+        As, Bs, Os, ref = [], [], [], []
+        for M, K, N in shapes:
+            a = rng.integers(-128, 127, (M, K), dtype=np.int8, endpoint=True)
+            b = rng.integers(-128, 127, (K, N), dtype=np.int8, endpoint=True)
+            As.append(ctx.from_numpy(a))
+            Bs.append(ctx.from_numpy(b))
+            Os.append(ctx.array((M, N), nc.DTYPE_OF_NP[np.dtype(odt)], zeros=True))
+            ref.append(oracle.matmul_int(a, b, odt))
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os, flags=flags), "batched int8")
+        for want, O in zip(ref, Os):
+            assert np.array_equal(O.numpy(), want)
+
+
+def test_batched_fallback_dtypes_and_empty(ctx):
+    rng = np.random.default_rng(18)
+    As, Bs, Os, ref = [], [], [], []
+    for M, K, N in [(30, 40, 50), (0, 4, 3), (5, 0, 6)]:
+        a, b = rng.standard_normal((M, K)), rng.standard_normal((K, N))
+        As.append(ctx.from_numpy(a))
+        Bs.append(ctx.from_numpy(b))
+        Os.append(ctx.from_numpy(np.full((M, N), 3.0)))
+        ref.append((a, b))
+    nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched f64")
+    assert cw_err(Os[0].numpy(), *ref[0]) <= 42 * U64
+    assert Os[1].numpy().shape == (0, 3)
+    assert np.all(Os[2].numpy() == 0)
