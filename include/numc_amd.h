@@ -104,6 +104,39 @@ typedef struct {
 
 NUMC_API int numc_kernel_config_count(int family);
 NUMC_API int numc_kernel_config_get(int family, int index, NumcKernelConfig *out);
+/* ---- autotuner and learned selector -----------------------------------------
+ * numc_autotune_shape times every configuration of `family` on device scratch
+ * operands of shape M x K @ K x N (flags: 1 A stored transposed, 2 B stored
+ * transposed, 4 padded rows), whole calls, mean of `reps`, into ms_out.
+ * The selector predicts log2(ms) per configuration with an MLP cost model and
+ * numc_matmul runs the argmin; without a trained model (or when it was
+ * trained for other configuration tables) an analytic model is used. */
+NUMC_API int numc_autotune_shape(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps,
+                                 float *ms_out, int ms_len);
+NUMC_API int numc_autotune_release(NumcCtx *ctx);
+NUMC_API int numc_selector_feature_count(void);
+NUMC_API int numc_selector_features(int family, int M, int N, int K, int flags, int cfg, float *out);
+NUMC_API int numc_selector_load(const char *path);
+NUMC_API int numc_selector_unload(void);
+NUMC_API int numc_selector_active(void);
+NUMC_API int numc_selector_predict(int family, int M, int N, int K, int flags, float *out, int len);
+/* the analytic model used without a trained selector */
+NUMC_API int numc_selector_heuristic(int family, int M, int N, int K);
+
+/* The cost model's network: 3 layers (din -> hidden -> hidden -> dout, ReLU),
+ * trained and evaluated by hand-written HIP kernels on the context's stream. */
+typedef struct NumcMlp NumcMlp;
+NUMC_API NumcMlp *numc_mlp_create(NumcCtx *ctx, int din, int hidden, int dout, uint64_t seed);
+NUMC_API void numc_mlp_free(NumcMlp *mlp);
+NUMC_API size_t numc_mlp_param_count(const NumcMlp *mlp);
+NUMC_API int numc_mlp_set_params(NumcMlp *mlp, const float *params, size_t count);
+NUMC_API int numc_mlp_get_params(NumcMlp *mlp, float *params, size_t count);
+NUMC_API int numc_mlp_forward(NumcMlp *mlp, const float *x, long long rows, float *y);
+NUMC_API int numc_mlp_loss_grad(NumcMlp *mlp, const float *x, const float *t, const float *mask, int batch,
+                                double *loss, float *grads);
+NUMC_API int numc_mlp_train(NumcMlp *mlp, const float *x, const float *t, const float *mask, long long rows,
+                            int epochs, int batch, float lr, float lr_final, uint64_t seed, double *loss_out);
+
 /* family a call with these arguments would run (or a negative error code) */
 NUMC_API int numc_matmul_family(const NumcArray *a, const NumcArray *b,
                                 const NumcArray *out, const NumcMatmulOpts *opts);

@@ -93,6 +93,7 @@ _SIGS = {
     "numc_matmul_batched": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                       C.POINTER(MatmulOpts)]),
     "numc_ctx_batch_graph_replays": (C.c_longlong, [_P]),
+    "numc_autotune_release": (C.c_int, [_P]),
     "numc_array_to_device": (C.c_int, [_P]),
     "numc_array_to_host": (C.c_int, [_P]),
     "numc_array_on_device": (C.c_int, [_P]),

@@ -51,6 +51,7 @@ struct NumcCtx {
   Workspace ws_aux;  // flags, scales, batched tables
   void *batch_cache; // BatchCache (batched.hip)
   void *prof;        // Profiler (runtime.cpp), null unless profiling is on
+  void *tune;        // autotuner scratch operands (autotune.hip)
 };
 
 // ---- kernel timing on the context stream (numc_ctx_profile) ----

@@ -125,6 +125,11 @@ static int heuristic_config(int family, int M, int N, int K) {
   return best;
 }
 
+extern "C" NUMC_API int numc_selector_heuristic(int family, int M, int N, int K) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "bad family %d", family);
+  return heuristic_config(family, M, N, K);
+}
+
 int numc_select_config(int family, int M, int N, int K, int flags) {
   int s = numc_selector_choose(family, M, N, K, flags);
   if (s >= 0 && s < numc_gemm_config_count(family)) return s;
@@ -168,7 +173,10 @@ int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, cons
   if (family == 100) {
     err = numc_i64_launch(ctx, p);
   } else {
-    int cfg_idx = opts && opts->config >= 0 ? opts->config : numc_select_config(family, p.M, p.N, p.K, 0);
+    // layout flags as the autotuner measures them: A stored K x M, B stored N x K, padded rows
+    const int lay = (p.csa != 1 ? 1 : 0) | (p.csb != 1 ? 2 : 0) |
+                    ((p.csa == 1 && p.rsa > p.K) || (p.csb == 1 && p.rsb > p.N) ? 4 : 0);
+    int cfg_idx = opts && opts->config >= 0 ? opts->config : numc_select_config(family, p.M, p.N, p.K, lay);
     const GemmConfig *cfg = numc_gemm_config(family, cfg_idx);
     if (!cfg) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: no config %d for family %d", cfg_idx, family);
     size_t aux = 0, need = numc_gemm_workspace(p, *cfg, &aux);

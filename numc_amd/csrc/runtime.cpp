@@ -167,6 +167,7 @@ int numc_finish(NumcCtx *ctx) {
 }
 
 extern "C" void numc_batch_cache_free(NumcCtx *ctx); // batched.hip
+extern "C" NUMC_API int numc_autotune_release(NumcCtx *ctx); // autotune.hip
 
 // ---------------------------------------------------------------------------
 // kernel timing: events recorded on the context stream around each launch
@@ -265,6 +266,7 @@ extern "C" NUMC_API void numc_ctx_free(NumcCtx *ctx) {
   if (!ctx) return;
   if (ctx->gpu_ready) (void)hipStreamSynchronize(ctx->stream);
   numc_batch_cache_free(ctx);
+  if (ctx->tune) numc_autotune_release(ctx);
   if (ctx->prof) delete (Profiler *)ctx->prof;
   for (ArenaBlock *b = ctx->head; b;) {
     ArenaBlock *n = b->next;

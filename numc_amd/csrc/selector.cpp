@@ -1,6 +1,228 @@
-// selector.cpp -- placeholder until the learned selector is loaded.
+// selector.cpp -- learned tile selector: picks a kernel configuration per
+// (family, shape, layout) with the MLP cost model trained on the autotuner's
+// timings (numc_amd/data/selector.bin), falling back to the analytic model
+// in matmul.cpp when no trained model matches the compiled configuration
+// tables.  numc's counterpart is its compile-time threshold table
+// (src/matmul/matmul.c:241-256) choosing gemmsup vs packed.
+//
+// Model input (must match numc_amd/autotune.py `features`):
+//   log2 M/16, log2 N/16, log2 K/16, family one-hot (5), transposed A,
+//   transposed B, padded rows, log2 bm/8, log2 bn/8, log2 kc/8,
+//   log2(waves)/3, log2(split)/3, log2(blocks)/16, log2(rounds of 256)/8
+// Model output: log2 of the predicted call time in ms.
 #include "gemm.h"
-int numc_selector_choose(int, int, int, int, int) { return -1; }
+#include "internal.h"
+
+#include <dlfcn.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+constexpr int kDin = 18;
+
+struct HostMlp {
+  int din = 0, H = 0, dout = 0;
+  std::vector<float> p;
+  float forward(const float *x) const {
+    std::vector<float> h1(H), h2(H);
+    const float *W0 = p.data(), *b0 = W0 + (size_t)din * H, *W1 = b0 + H, *b1 = W1 + (size_t)H * H, *W2 = b1 + H,
+                *b2 = W2 + (size_t)H * dout;
+    for (int j = 0; j < H; j++) {
+      float s = b0[j];
+      for (int k = 0; k < din; k++) s = std::fma(x[k], W0[k * H + j], s);
+      h1[j] = s > 0 ? s : 0;
+    }
+    for (int j = 0; j < H; j++) {
+      float s = b1[j];
+      for (int k = 0; k < H; k++) s = std::fma(h1[k], W1[k * H + j], s);
+      h2[j] = s > 0 ? s : 0;
+    }
+    float s = b2[0];
+    for (int k = 0; k < H; k++) s = std::fma(h2[k], W2[k * dout], s);
+    return s;
+  }
+};
+
+std::mutex g_mu;
+HostMlp g_model;
+bool g_tried = false, g_loaded = false;
+struct ShapeKey {
+  int family, flags, M, N, K;
+  bool operator==(const ShapeKey &o) const {
+    return family == o.family && flags == o.flags && M == o.M && N == o.N && K == o.K;
+  }
+};
+struct ShapeKeyHash {
+  size_t operator()(const ShapeKey &k) const {
+    uint64_t h = 1469598103934665603ULL;
+    for (int v : {k.family, k.flags, k.M, k.N, k.K}) h = (h ^ (uint32_t)v) * 1099511628211ULL;
+    return (size_t)h;
+  }
+};
+std::unordered_map<ShapeKey, int, ShapeKeyHash> g_cache;
+
+std::string default_path() {
+  Dl_info info;
+  if (dladdr((void *)&numc_select_config, &info) && info.dli_fname) {
+    std::string so = info.dli_fname;
+    size_t s = so.rfind('/');
+    std::string dir = s == std::string::npos ? "." : so.substr(0, s);
+    return dir + "/../data/selector.bin";
+  }
+  return "numc_amd/data/selector.bin";
+}
+
+// file: "NMLS", version 1, din, hidden, dout, nfam, per family {ncfg, ncfg x 6 ints}, float params
+bool load_file(const char *path, HostMlp *m, char *why, size_t whylen) {
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    snprintf(why, whylen, "cannot open %s", path);
+    return false;
+  }
+  char magic[4];
+  int hdr[5];
+  bool ok = fread(magic, 1, 4, f) == 4 && memcmp(magic, "NMLS", 4) == 0 && fread(hdr, sizeof(int), 5, f) == 5 &&
+            hdr[0] == 1 && hdr[1] == kDin && hdr[3] == 1 && hdr[4] == NUMC_FAMILY_COUNT;
+  if (!ok) {
+    snprintf(why, whylen, "bad header");
+    fclose(f);
+    return false;
+  }
+  for (int fam = 0; fam < NUMC_FAMILY_COUNT && ok; fam++) {
+    int n = 0;
+    ok = fread(&n, sizeof(int), 1, f) == 1 && n == numc_gemm_config_count(fam);
+    for (int i = 0; ok && i < n; i++) {
+      int d[6];
+      ok = fread(d, sizeof(int), 6, f) == 6;
+      const NumcKernelConfig &c = numc_gemm_config(fam, i)->cfg;
+      ok = ok && d[0] == c.bm && d[1] == c.bn && d[2] == c.kc && d[3] == c.waves_m && d[4] == c.waves_n &&
+           d[5] == c.split_k;
+    }
+  }
+  if (!ok) {
+    snprintf(why, whylen, "configuration tables differ from the trained model");
+    fclose(f);
+    return false;
+  }
+  m->din = hdr[1];
+  m->H = hdr[2];
+  m->dout = hdr[3];
+  size_t np = (size_t)m->din * m->H + m->H + (size_t)m->H * m->H + m->H + (size_t)m->H * m->dout + m->dout;
+  m->p.resize(np);
+  ok = fread(m->p.data(), sizeof(float), np, f) == np;
+  fclose(f);
+  if (!ok) snprintf(why, whylen, "truncated parameters");
+  return ok;
+}
+
+void ensure_loaded() {
+  if (g_tried) return;
+  g_tried = true;
+  const char *env = getenv("NUMC_SELECTOR");
+  if (env && strcmp(env, "off") == 0) return;
+  std::string path = env && *env ? env : default_path();
+  char why[256];
+  g_loaded = load_file(path.c_str(), &g_model, why, sizeof why);
+}
+} // namespace
+
+// features of (family, shape, layout flags, config) -- see the header comment
+extern "C" NUMC_API int numc_selector_features(int family, int M, int N, int K, int flags, int cfg, float *out) {
+  const GemmConfig *c = numc_gemm_config(family, cfg);
+  if (!c || !out) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_features: no config %d in family %d", cfg, family);
+  const double m = M > 0 ? M : 1, n = N > 0 ? N : 1, k = K > 0 ? K : 1;
+  const double blocks = std::ceil(m / c->cfg.bm) * std::ceil(n / c->cfg.bn) * c->cfg.split_k;
+  out[0] = (float)(std::log2(m) / 16);
+  out[1] = (float)(std::log2(n) / 16);
+  out[2] = (float)(std::log2(k) / 16);
+  for (int f = 0; f < NUMC_FAMILY_COUNT; f++) out[3 + f] = f == family ? 1.f : 0.f;
+  out[8] = (flags & 1) ? 1.f : 0.f;
+  out[9] = (flags & 2) ? 1.f : 0.f;
+  out[10] = (flags & 4) ? 1.f : 0.f;
+  out[11] = (float)(std::log2((double)c->cfg.bm) / 8);
+  out[12] = (float)(std::log2((double)c->cfg.bn) / 8);
+  out[13] = (float)(std::log2((double)c->cfg.kc) / 8);
+  out[14] = (float)(std::log2((double)c->cfg.waves_m * c->cfg.waves_n) / 3);
+  out[15] = (float)(std::log2((double)c->cfg.split_k) / 3);
+  out[16] = (float)(std::log2(blocks) / 16);
+  out[17] = (float)(std::log2(std::ceil(blocks / 256.0)) / 8);
+  return 0;
+}
+
+extern "C" NUMC_API int numc_selector_feature_count(void) { return kDin; }
+
+// (re)load a trained selector; path NULL = the default next to the library
+extern "C" NUMC_API int numc_selector_load(const char *path) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  HostMlp m;
+  char why[256];
+  std::string p = path ? path : default_path();
+  if (!load_file(p.c_str(), &m, why, sizeof why)) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_load: %s", why);
+  g_model = m;
+  g_loaded = true;
+  g_tried = true;
+  g_cache.clear();
+  return 0;
+}
+
+extern "C" NUMC_API int numc_selector_unload(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_loaded = false;
+  g_tried = true;
+  g_cache.clear();
+  return 0;
+}
+
+extern "C" NUMC_API int numc_selector_active(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_loaded();
+  return g_loaded ? 1 : 0;
+}
+
+// predicted log2(ms) of every config of a family (host inference)
+extern "C" NUMC_API int numc_selector_predict(int family, int M, int N, int K, int flags, float *out, int len) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_loaded();
+  if (!g_loaded) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_predict: no trained selector loaded");
+  const int n = numc_gemm_config_count(family);
+  if (len < n) return NUMC_SET_ERROR(NUMC_ERR_BOUNDS, "selector_predict: need %d slots", n);
+  float x[kDin];
+  for (int i = 0; i < n; i++) {
+    numc_selector_features(family, M, N, K, flags, i, x);
+    out[i] = g_model.forward(x);
+  }
+  return 0;
+}
+
+int numc_selector_choose(int family, int M, int N, int K, int flags) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_loaded();
+  if (!g_loaded) return -1;
+  const ShapeKey key{family, flags, M, N, K};
+  auto it = g_cache.find(key);
+  if (it != g_cache.end()) return it->second;
+  const int n = numc_gemm_config_count(family);
+  float x[kDin];
+  int best = -1;
+  float bt = 0;
+  for (int i = 0; i < n; i++) {
+    numc_selector_features(family, M, N, K, flags, i, x);
+    float t = g_model.forward(x);
+    if (best < 0 || t < bt) {
+      best = i;
+      bt = t;
+    }
+  }
+  if (g_cache.size() > 65536) g_cache.clear();
+  g_cache[key] = best;
+  return best;
+}
 
 // tile class of a problem in the persistent batched kernel: 0 = 128x128, 1 = 64x64
 int numc_batch_class(int family, int M, int N, int K) {
