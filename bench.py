@@ -38,7 +38,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_method(nc, ctx, A, B, C, method, family, steps, warmup):
+def time_method(nc, ctx, A, B, C, method, family, steps, warmup, n):
+    # exhaustive autotune of this exact shape (the selector's fallback): the
+    # timed calls then run the measured best configuration
+    cfg = nc.lib().numc_selector_tune(ctx.h, family, n, n, n, 0, 5)
+    if cfg < 0:
+        nc.check(cfg, "numc_selector_tune")
     for _ in range(warmup):
         nc.check(nc.matmul(A, B, C, method), "warmup")
     ctx.synchronize()
@@ -56,7 +61,7 @@ def time_method(nc, ctx, A, B, C, method, family, steps, warmup):
     gemm_ms, gemm_n = ctx.profile_read(1, family)
     pack_ms, pack_n = ctx.profile_read(0, family)
     ctx.profile(False)
-    return {"ms_per_step": region / steps, "wall_ms_per_step": wall / steps,
+    return {"config": nc.config_get(family, cfg), "ms_per_step": region / steps, "wall_ms_per_step": wall / steps,
             "kernel_ms": gemm_ms / max(gemm_n, 1), "kernel_launches": gemm_n,
             "pack_ms": pack_ms / max(pack_n, 1)}
 
@@ -145,6 +150,19 @@ def bench_batched(nc, count, seed, reps):
     return res
 
 
+def read_traffic():
+    """HBM bytes per launch of the split-BF16 kernel from the committed rocprof
+    summary (FETCH_SIZE/WRITE_SIZE runs scaled by a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "gemm_split_bf16_8192_traffic.txt")
+    try:
+        txt = open(path).read()
+        d = json.loads(txt[txt.index("{"):])
+        return {"hbm_bytes_per_launch": round(d["hbm_bytes_per_launch"]),
+                "source": "profiles/gemm_split_bf16_8192_traffic.txt (%s)" % d["kernel"]}
+    except (OSError, ValueError, KeyError):
+        return {}
+
+
 def wsum(x):
     w = np.ascontiguousarray(x).view(np.uint32).ravel().astype(np.uint64)
     return int(np.sum(w * np.arange(1, w.size + 1, dtype=np.uint64)))
@@ -161,6 +179,9 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--only", default=None, help="time one method: split_bf16, f32 or fp8")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the reference copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
@@ -168,6 +189,19 @@ def main():
         sys.exit(2)
 
     import numc_amd as nc
+    if args.copy_probe:
+        ctx = nc.Ctx()
+        nbytes = args.copy_probe << 20
+        X = ctx.array((nbytes,), nc.UINT8)
+        Y = ctx.array((nbytes,), nc.UINT8)
+        X.to_device()
+        Y.to_device()
+        src, dst = nc.lib().numc_array_device_data(X.h), nc.lib().numc_array_device_data(Y.h)
+        for _ in range(3):
+            nc.check(nc.lib().numc_device_copy(ctx.h, dst, src, nbytes), "numc_device_copy")
+        print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
+        ctx.close()
+        return
     n = args.n
     flops = 2.0 * n * n * n
     ctx = nc.Ctx()
@@ -190,9 +224,11 @@ def main():
     plan = [("split_bf16", nc.METHOD_SPLIT_BF16, nc.FAMILY_SPLIT_BF16),
             ("f32", nc.METHOD_PLAIN, nc.FAMILY_F32),
             ("fp8", nc.METHOD_FP8_SCALED, nc.FAMILY_FP8)]
+    if args.only:
+        plan = [x for x in plan if x[0] == args.only]
     for name, method, fam in plan:
         log("bench: timing %s (%d warmup, %d steps)" % (name, args.warmup, args.steps))
-        t = time_method(nc, ctx, A, B, C[name], method, fam, args.steps, args.warmup)
+        t = time_method(nc, ctx, A, B, C[name], method, fam, args.steps, args.warmup, n)
         t["tflops"] = flops / t["ms_per_step"] * 1e-9
         t["kernel_tflops"] = flops / t["kernel_ms"] * 1e-9
         t["peak_tflops"] = PEAK[name]
@@ -200,6 +236,12 @@ def main():
         variants[name] = t
         log("bench: %s %.3f ms/step, %.1f TFLOP/s (kernel %.3f ms, %.1f TFLOP/s)" %
             (name, t["ms_per_step"], t["tflops"], t["kernel_ms"], t["kernel_tflops"]))
+
+    if args.only:
+        print(json.dumps({"only": args.only, "n": n, **{k: (float("%.5g" % v) if isinstance(v, float) else v)
+                                                          for k, v in variants[args.only].items()}}), flush=True)
+        ctx.close()
+        return
 
     errs = None
     if not args.no_errors:
@@ -246,6 +288,7 @@ def main():
 
     s = variants["split_bf16"]
     p = variants["f32"]
+    traffic = read_traffic()
     out = {
         "metric": METRIC,
         "value": round(s["tflops"], 3),
@@ -264,9 +307,12 @@ def main():
                                % n, "M": n, "N": n, "K": n, "method": "split_bf16"},
         "roofline": {"bound": "mfma", "achieved": round(s["kernel_tflops"], 3), "peak": round(PEAK["split_bf16"], 1),
                      "unit": "TFLOP/s", "frac": round(s["kernel_tflops"] / PEAK["split_bf16"], 4),
-                     "traffic": None,
-                     "kernel": "k_gemm<MfBF16,...,MODE_SPLIT3> (avg %.4f ms over %d launches)"
-                               % (s["kernel_ms"], s["kernel_launches"])},
+                     "traffic": traffic.get("hbm_bytes_per_launch"),
+                     "traffic_source": traffic.get("source"),
+                     "kernel": "split-BF16 MFMA kernel, tile %(bm)dx%(bn)d k%(kc)d waves %(waves_m)dx%(waves_n)d"
+                               % s["config"] + " (avg %.4f ms over %d launches)" % (s["kernel_ms"], s["kernel_launches"]),
+                     "algorithmic": "2*M*N*K = %.4g FP32-equivalent flop per launch; peak = 2516/3 TFLOP/s "
+                                    "(three bf16 products per FP32 product)" % flops},
         "cpu_baseline": cpu,
         "split_vs_plain_speedup": round(s["tflops"] / p["tflops"], 3),
         "variants": {k: {kk: (float("%.5g" % vv) if isinstance(vv, float) else vv) for kk, vv in v.items()}

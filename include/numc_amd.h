@@ -78,6 +78,10 @@ NUMC_API int numc_ctx_profile(NumcCtx *ctx, int enable);
 NUMC_API int numc_ctx_profile_read(NumcCtx *ctx, int category, int family, double *total_ms,
                                    long long *launches, int reset);
 
+/* plain device-to-device copy by a simple vector kernel on the context stream
+ * (the known-traffic reference used to calibrate HBM counters) */
+NUMC_API int numc_device_copy(NumcCtx *ctx, void *dst, const void *src, size_t bytes);
+
 /* ---- seeded inputs ----------------------------------------------------------
  * Fills a contiguous array with the documented generator: element i comes from
  * xoshiro256** block i>>16 seeded by splitmix64 from seed, stream and block.
@@ -120,6 +124,11 @@ NUMC_API int numc_selector_load(const char *path);
 NUMC_API int numc_selector_unload(void);
 NUMC_API int numc_selector_active(void);
 NUMC_API int numc_selector_predict(int family, int M, int N, int K, int flags, float *out, int len);
+/* exhaustive autotune of one exact shape on this device: its best
+ * configuration is used for that shape from then on (returns the index) */
+NUMC_API int numc_selector_tune(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps);
+/* pin (cfg >= 0) or unpin (cfg < 0) the configuration of an exact shape */
+NUMC_API int numc_selector_pin(int family, int M, int N, int K, int flags, int cfg);
 /* the analytic model used without a trained selector */
 NUMC_API int numc_selector_heuristic(int family, int M, int N, int K);
 

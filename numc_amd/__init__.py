@@ -94,6 +94,10 @@ _SIGS = {
                                       C.POINTER(MatmulOpts)]),
     "numc_ctx_batch_graph_replays": (C.c_longlong, [_P]),
     "numc_autotune_release": (C.c_int, [_P]),
+    "numc_device_copy": (C.c_int, [_P, _P, _P, _SZ]),
+    "numc_selector_tune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "numc_selector_pin": (C.c_int, [C.c_int] * 6),
+    "numc_selector_active": (C.c_int, []),
     "numc_array_to_device": (C.c_int, [_P]),
     "numc_array_to_host": (C.c_int, [_P]),
     "numc_array_on_device": (C.c_int, [_P]),

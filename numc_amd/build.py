@@ -30,9 +30,7 @@ DEVICE = [
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + DEVICE
-    if src.endswith(".hip"):
-        cmd += ["-x", "hip"]
+    cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip"]
     cmd += ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines()

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <thread>
 #include <vector>
 
@@ -768,6 +769,27 @@ extern "C" NUMC_API void *numc_array_device_data(const NumcArray *arr) {
     return nullptr;
   }
   return numc_dev_ptr(arr);
+}
+
+// ---------------------------------------------------------------------------
+// plain device copy (16-byte vectors, grid-stride): the reference kernel of
+// known traffic that HBM counters are calibrated against
+// ---------------------------------------------------------------------------
+
+__global__ void k_copy16(const uint4 *src, uint4 *dst, long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
+}
+
+extern "C" NUMC_API int numc_device_copy(NumcCtx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || !dst || !src) return NUMC_SET_ERROR(NUMC_ERR_NULL, "device_copy: NULL argument");
+  if (bytes % 16) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "device_copy: size must be a multiple of 16");
+  int err = numc_gpu_init(ctx);
+  if (err) return err;
+  long long n = (long long)(bytes / 16);
+  unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 2048);
+  if (n) hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)src, (uint4 *)dst, n);
+  HIP_TRY(hipGetLastError(), "device_copy launch");
+  return numc_finish(ctx);
 }
 
 // ---------------------------------------------------------------------------

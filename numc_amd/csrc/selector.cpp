@@ -66,6 +66,10 @@ struct ShapeKeyHash {
   }
 };
 std::unordered_map<ShapeKey, int, ShapeKeyHash> g_cache;
+// exact shapes whose best configuration was measured (exhaustive autotune):
+// consulted before the model
+std::unordered_map<ShapeKey, int, ShapeKeyHash> g_tuned;
+bool g_tuned_tried = false;
 
 std::string default_path() {
   Dl_info info;
@@ -121,7 +125,36 @@ bool load_file(const char *path, HostMlp *m, char *why, size_t whylen) {
   return ok;
 }
 
+std::string default_tuned_path() {
+  std::string p = default_path();
+  return p.substr(0, p.rfind('/')) + "/tuned.csv";
+}
+
+// lines "family,M,N,K,flags,cfg"; '#' starts a comment
+void load_tuned(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int fam, M, N, K, fl, cfg;
+    if (sscanf(line, "%d,%d,%d,%d,%d,%d", &fam, &M, &N, &K, &fl, &cfg) == 6 && cfg >= 0 &&
+        cfg < numc_gemm_config_count(fam))
+      g_tuned[ShapeKey{fam, fl, M, N, K}] = cfg;
+  }
+  fclose(f);
+}
+
+void ensure_tuned() {
+  if (g_tuned_tried) return;
+  g_tuned_tried = true;
+  const char *env = getenv("NUMC_SELECTOR");
+  if (env && strcmp(env, "off") == 0) return;
+  load_tuned(default_tuned_path().c_str());
+}
+
 void ensure_loaded() {
+  ensure_tuned();
   if (g_tried) return;
   g_tried = true;
   const char *env = getenv("NUMC_SELECTOR");
@@ -171,11 +204,41 @@ extern "C" NUMC_API int numc_selector_load(const char *path) {
   return 0;
 }
 
+// exhaustive autotune of one shape on this device; the best configuration is
+// used for that exact shape from then on (returns its index)
+extern "C" NUMC_API int numc_autotune_shape(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps,
+                                            float *ms_out, int ms_len);
+extern "C" NUMC_API int numc_selector_tune(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps) {
+  float ms[32];
+  int err = numc_autotune_shape(ctx, family, M, N, K, flags, reps, ms, 32);
+  if (err) return err;
+  int best = 0;
+  for (int i = 1; i < numc_gemm_config_count(family); i++)
+    if (ms[i] < ms[best]) best = i;
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_tuned();
+  g_tuned[ShapeKey{family, flags, M, N, K}] = best;
+  return best;
+}
+
+// pin a configuration for an exact shape (cfg < 0 removes the pin)
+extern "C" NUMC_API int numc_selector_pin(int family, int M, int N, int K, int flags, int cfg) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT || cfg >= numc_gemm_config_count(family))
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_pin: bad family/config");
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_tuned();
+  if (cfg < 0) g_tuned.erase(ShapeKey{family, flags, M, N, K});
+  else g_tuned[ShapeKey{family, flags, M, N, K}] = cfg;
+  return 0;
+}
+
 extern "C" NUMC_API int numc_selector_unload(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_loaded = false;
   g_tried = true;
   g_cache.clear();
+  g_tuned.clear();
+  g_tuned_tried = true;
   return 0;
 }
 
@@ -203,8 +266,10 @@ extern "C" NUMC_API int numc_selector_predict(int family, int M, int N, int K, i
 int numc_selector_choose(int family, int M, int N, int K, int flags) {
   std::lock_guard<std::mutex> lk(g_mu);
   ensure_loaded();
-  if (!g_loaded) return -1;
   const ShapeKey key{family, flags, M, N, K};
+  auto tt = g_tuned.find(key);
+  if (tt != g_tuned.end()) return tt->second;
+  if (!g_loaded) return -1;
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return it->second;
   const int n = numc_gemm_config_count(family);
