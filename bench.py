@@ -7,6 +7,9 @@ the plain FP32 and the per-row/column-scaled FP8 methods are timed the same
 way and reported beside it.  value = 2*M*N*K / ms_per_step (FP32-equivalent
 TFLOP/s; extra MFMA products are not counted as work).
 
+Configuration: the one numc_matmul runs for the shape (the committed
+exact-shape table numc_amd/data/tuned.csv, from exhaustive autotune).
+
 Errors: max componentwise |C - R| / (|A||B|) over all 67M entries, with R and
 |A||B| from the library's own FP64 MFMA GEMM (validated against the CPU oracle
 by the test suite), for the N(0,1) inputs and an ill-scaled set (rows of A and
@@ -39,11 +42,12 @@ def log(*a):
 
 
 def time_method(nc, ctx, A, B, C, method, family, steps, warmup, n):
-    # exhaustive autotune of this exact shape (the selector's fallback): the
-    # timed calls then run the measured best configuration
-    cfg = nc.lib().numc_selector_tune(ctx.h, family, n, n, n, 0, 5)
+    # the configuration numc_matmul runs for this shape: the committed
+    # exact-shape table (numc_amd/data/tuned.csv, made by exhaustive autotune
+    # with `python -m numc_amd.autotune pin`), else the learned selector
+    cfg = nc.lib().numc_selector_choice(family, n, n, n, 0)
     if cfg < 0:
-        nc.check(cfg, "numc_selector_tune")
+        nc.check(cfg, "numc_selector_choice")
     for _ in range(warmup):
         nc.check(nc.matmul(A, B, C, method), "warmup")
     ctx.synchronize()
@@ -178,6 +182,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--only", default=None, help="time one method: split_bf16, f32 or fp8")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -281,6 +286,23 @@ def main():
         else:
             cpu = {"value": None, "unit": "TFLOP/s", "cores": cores, "kind": "reference", "sample": why}
 
+    # numc's default residency: host arrays, each call uploads A and B and
+    # downloads C (PCIe-inclusive; reported beside, never as the value)
+    pcie = None
+    if not args.no_pcie:
+        ctx_h = nc.Ctx()
+        AH, BH = ctx_h.from_numpy(a_host), ctx_h.from_numpy(b_host)
+        CH = ctx_h.array((n, n), nc.FLOAT32, zeros=True)
+        nc.check(nc.matmul(AH, BH, CH, nc.METHOD_SPLIT_BF16), "host-resident warmup")
+        t0 = time.perf_counter()
+        for _ in range(3):
+            nc.check(nc.matmul(AH, BH, CH, nc.METHOD_SPLIT_BF16), "host-resident")
+        ms = (time.perf_counter() - t0) * 1e3 / 3
+        pcie = {"ms_per_call": round(ms, 3), "tflops": round(flops / ms * 1e-9, 2),
+                "note": "split-BF16, host-resident arrays (H2D of A and B, D2H of C each call), synchronous"}
+        log("bench: host-resident split-BF16 call %.2f ms (%.1f TFLOP/s incl. PCIe)" % (ms, flops / ms * 1e-9))
+        ctx_h.close()
+
     batched = None
     if not args.no_batched:
         log("bench: batched uneven GEMM, %d problems" % args.batch)
@@ -320,6 +342,8 @@ def main():
     }
     if batched:
         out["batched"] = batched
+    if pcie:
+        out["host_resident_pcie_inclusive"] = pcie
     if errs:
         out["err_ratio_split_vs_plain"] = round(errs["normal"]["split_bf16"]["componentwise"] /
                                                 errs["normal"]["f32"]["componentwise"], 4)

@@ -28,7 +28,8 @@ typedef struct {
   int flags;    /* NUMC_OPT_* */
 } NumcMatmulOpts;
 
-#define NUMC_OPT_WIDEN 1 /* allow int8/uint8 inputs with an int32/uint32 output */
+#define NUMC_OPT_WIDEN 1          /* allow int8/uint8 inputs with an int32/uint32 output */
+#define NUMC_OPT_DEVICE_POINTERS 2 /* numc_amd_gemm: a, b, c are device addresses */
 
 /* numc_matmul with a method and configuration (no counterpart in numc; the
  * default options give exactly numc_matmul).  With NUMC_OPT_WIDEN, a and b
@@ -43,6 +44,20 @@ NUMC_API int numc_matmul_ex(const NumcArray *a, const NumcArray *b, NumcArray *o
 NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArray *const *a,
                                  NumcArray *const *b, NumcArray *const *out,
                                  const NumcMatmulOpts *opts);
+
+/* Pointer-level GEMM for a host library that keeps its own arrays: the call
+ * numc's numc_matmul (src/matmul/matmul.c:347-389) makes to its dtype kernel
+ * table (matmul.c:210-320, kernels taking data pointers and strides) becomes
+ * this one.  c (M x N) = a (M x K) @ b (K x N); strides are in bytes, as in
+ * numc's NumcArray (src/internal.h:35-43), and every one is honoured.  a, b
+ * and c are host pointers (staged through the context's device workspace,
+ * gaps of a strided c preserved), or device addresses with
+ * NUMC_OPT_DEVICE_POINTERS.  Result contract, methods and errors as
+ * numc_matmul_ex (dtype == out_dtype unless NUMC_OPT_WIDEN).  The context
+ * comes from numc_ctx_create; no arrays of it are needed. */
+NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, size_t M, size_t N, size_t K,
+                           const void *a, size_t rsa, size_t csa, const void *b, size_t rsb, size_t csb,
+                           void *c, size_t rso, size_t cso, const NumcMatmulOpts *opts);
 
 /* number of times the context replayed its cached batched graph */
 NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx);
@@ -131,6 +146,9 @@ NUMC_API int numc_selector_tune(NumcCtx *ctx, int family, int M, int N, int K, i
 NUMC_API int numc_selector_pin(int family, int M, int N, int K, int flags, int cfg);
 /* the analytic model used without a trained selector */
 NUMC_API int numc_selector_heuristic(int family, int M, int N, int K);
+/* the configuration numc_matmul would run for this shape: exact-shape table,
+ * then the learned selector, then the analytic model */
+NUMC_API int numc_selector_choice(int family, int M, int N, int K, int flags);
 
 /* The cost model's network: 3 layers (din -> hidden -> hidden -> dout, ReLU),
  * trained and evaluated by hand-written HIP kernels on the context's stream. */

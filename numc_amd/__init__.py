@@ -28,6 +28,7 @@ METHOD_AUTO, METHOD_PLAIN, METHOD_SPLIT_BF16, METHOD_FP8_SCALED = 0, 1, 2, 3
 FAMILY_F32, FAMILY_SPLIT_BF16, FAMILY_FP8, FAMILY_F64, FAMILY_I8 = range(5)
 FAMILY_NAMES = ["f32", "split_bf16", "fp8", "f64", "i8"]
 OPT_WIDEN = 1
+OPT_DEVICE_POINTERS = 2
 DIST_UNIFORM, DIST_NORMAL, DIST_INT = 0, 1, 2
 
 
@@ -90,6 +91,9 @@ _SIGS = {
     "numc_matmul_naive": (C.c_int, [_P, _P, _P]),
     # numc_amd.h
     "numc_matmul_ex": (C.c_int, [_P, _P, _P, C.POINTER(MatmulOpts)]),
+    "numc_amd_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t,
+                                _P, C.c_size_t, C.c_size_t, _P, C.c_size_t, C.c_size_t,
+                                _P, C.c_size_t, C.c_size_t, C.POINTER(MatmulOpts)]),
     "numc_matmul_batched": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                       C.POINTER(MatmulOpts)]),
     "numc_ctx_batch_graph_replays": (C.c_longlong, [_P]),
@@ -283,6 +287,19 @@ class Array:
 def matmul(a, b, out, method=METHOD_AUTO, config=-1, flags=0):
     opts = MatmulOpts(method, config, flags)
     return lib().numc_matmul_ex(a.h, b.h, out.h, C.byref(opts))
+
+
+def gemm_pointers(ctx, a, b, c, method=METHOD_AUTO, config=-1, flags=0):
+    """numc_amd_gemm on numpy arrays (host pointers; byte strides taken from
+    the arrays, so strided views of a, b and c work as they are)."""
+    dt, odt = DTYPE_OF_NP[a.dtype], DTYPE_OF_NP[c.dtype]
+    M, K = a.shape
+    N = b.shape[1]
+    opts = MatmulOpts(method, config, flags)
+    return lib().numc_amd_gemm(ctx.h, dt, odt, M, N, K,
+                               a.ctypes.data, a.strides[0], a.strides[1],
+                               b.ctypes.data, b.strides[0], b.strides[1],
+                               c.ctypes.data, c.strides[0], c.strides[1], C.byref(opts))
 
 
 def matmul_batched(ctx, As, Bs, Outs, method=METHOD_AUTO, config=-1, flags=0):

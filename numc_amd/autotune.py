@@ -64,6 +64,8 @@ def _setup():
     L.numc_selector_load.argtypes = [C.c_char_p]
     L.numc_selector_heuristic.argtypes = [C.c_int] * 4
     L.numc_selector_heuristic.restype = C.c_int
+    L.numc_selector_choice.argtypes = [C.c_int] * 5
+    L.numc_selector_choice.restype = C.c_int
     L.numc_mlp_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64]
     L.numc_mlp_create.restype = C.c_void_p
     L.numc_mlp_free.argtypes = [C.c_void_p]
@@ -241,9 +243,42 @@ def evaluate(db_path, model_path, idxs=None, use_gpu=False):
             "mlp_mean_regret_by_family": {k: float(np.mean(v)) for k, v in by_family.items()}}
 
 
+def pin(specs, out, reps=10, rounds=3, log=print):
+    """Exhaustively autotune exact shapes and record the winners in the
+    exact-shape table (lines family,M,N,K,flags,cfg): the per-configuration
+    time is the minimum over `rounds` rounds of `reps` calls each."""
+    L = _setup()
+    ctx = Ctx()
+    buf = (C.c_float * 16)()
+    rows = {}
+    if os.path.exists(out):
+        for line in open(out):
+            if line.strip() and not line.startswith("#"):
+                f = line.strip().split(",")
+                rows[tuple(int(x) for x in f[:5])] = int(f[5])
+    for fam, M, N, K, fl in specs:
+        n = config_count(fam)
+        best = [float("inf")] * n
+        for _ in range(rounds):
+            check(L.numc_autotune_shape(ctx.h, fam, M, N, K, fl, reps, buf, 16), "numc_autotune_shape")
+            best = [min(best[c], buf[c]) for c in range(n)]
+        cfg = int(np.argmin(best))
+        rows[(fam, M, N, K, fl)] = cfg
+        log("pin: family %d %dx%dx%d flags %d -> config %d (%s)" % (
+            fam, M, N, K, fl, cfg, " ".join("%.4f" % t for t in best)), flush=True)
+    lib().numc_autotune_release(ctx.h)
+    ctx.close()
+    with open(out, "w") as f:
+        f.write("# exact-shape configuration table: family,M,N,K,flags,cfg (python -m numc_amd.autotune pin)\n")
+        for k in sorted(rows):
+            f.write("%d,%d,%d,%d,%d,%d\n" % (k + (rows[k],)))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["sweep", "train", "eval"])
+    ap.add_argument("cmd", choices=["sweep", "train", "eval", "pin"])
+    ap.add_argument("--shapes", default="1:8192:8192:8192:0,0:8192:8192:8192:0,2:8192:8192:8192:0",
+                    help="pin: family:M:N:K:flags,...")
     ap.add_argument("--out", default=None)
     ap.add_argument("--db", default=DEFAULT_DB)
     ap.add_argument("--model", default=DEFAULT_MODEL)
@@ -255,6 +290,9 @@ def main():
         sweep(a.out or a.db, a.start, a.stop)
     elif a.cmd == "train":
         train(a.db, a.out or a.model, epochs=a.epochs)
+    elif a.cmd == "pin":
+        specs = [tuple(int(x) for x in s.split(":")) for s in a.shapes.split(",")]
+        pin(specs, a.out or os.path.join(os.path.dirname(DEFAULT_MODEL), "tuned.csv"))
     else:
         import json
         print(json.dumps(evaluate(a.db, a.model), indent=1))

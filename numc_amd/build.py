@@ -52,7 +52,10 @@ def build(verbose=False):
             print(msgs)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + \
-        [o for o, _ in results] + ["-lpthread"]
+        [o for o, _ in results] + ["-lpthread",
+                                   # calls between the library's own numc_* functions bind
+                                   # inside it, even in a process that also has numc loaded
+                                   "-Wl,-Bsymbolic"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 static inline unsigned blocks_for(long long n) { return (unsigned)cdiv(n, 256); }
@@ -271,14 +272,14 @@ __global__ void __launch_bounds__(256) k_pack_f32_tiled_rows(PackArgs a) {
   constexpr int TR = 128, TK = 32, PITCH = TK + 1;
   __shared__ float tile[TR * PITCH];
   constexpr int KF = MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32;
-  const int nkt = (a.nKF * KF + TK - 1) / TK;
-  const int nrt4 = (a.nRT + 3) / 4;
-  const long long rb = blockIdx.x / nkt;   // block of 4 row fragments
-  const int kt = blockIdx.x % nkt;
+  // consecutive blocks take neighbouring row blocks at one k range, so the
+  // segments they read complete each other's source rows
+  const int nrb = (a.nRT + 3) / 4;
+  const long long rb = blockIdx.x % nrb;   // block of 4 row fragments
+  const int kt = blockIdx.x / nrb;
   const int tid = threadIdx.x;
   const long long row0 = rb * TR;
   const int k0 = kt * TK;
-  (void)nrt4;
   // 32 k x 128 rows = 1024 float4 along rows, 4 per thread
 #pragma unroll
   for (int it = 0; it < 4; it++) {
@@ -362,6 +363,15 @@ __global__ void k_pack_f64(PackArgs a) {
 // integer -> P signed base-256 digit planes (T=32, 16 per lane, KF=32).
 // x = sum_p d_p 256^p (mod 256^P) with d_p in [-128, 127]; flags[0] gets a
 // bit per plane that holds a nonzero digit.
+// OR a wave's nonzero-plane mask into the shared flags word.  Thousands of
+// waves target the one word, and same-address atomics serialise in L2, so
+// the word is read first (from L2) and the atomic issued only when it would
+// set a new bit; a stale read only costs a redundant atomic.
+__device__ inline void flags_or(int *flags, int mask) {
+  const int cur = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (mask & ~cur) atomicOr(flags, mask);
+}
+
 template <class S, int P>
 __global__ void k_pack_digits(PackArgs a) {
   long long t = blockIdx.x * 256LL + threadIdx.x;
@@ -395,7 +405,7 @@ __global__ void k_pack_digits(PackArgs a) {
 #pragma unroll
   for (int p = 0; p < P; p++)
     if (__ballot((w[p][0] | w[p][1] | w[p][2] | w[p][3]) != 0)) mask |= 1 << p;
-  if ((threadIdx.x & 63) == 0 && mask) atomicOr(a.flags, mask);
+  if ((threadIdx.x & 63) == 0 && mask) flags_or(a.flags, mask);
 }
 
 // ---- FP8 scales: row maxima of |x| as float bits, then exponents ----------
@@ -449,15 +459,44 @@ typedef __attribute__((address_space(1))) void global_void_t;
 // vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] at [15:14])
 #define WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
 
-template <int N> __device__ inline void wait_vmcnt_upto(int n) {
-  // n is wave-uniform and at most N; s_waitcnt needs an immediate
-  if constexpr (N >= 0) {
-    if (n >= N) { WAIT_VMCNT(N); return; }
-    wait_vmcnt_upto<N - 1>(n);
-  } else {
-    WAIT_VMCNT(0);
-  }
+// Workgroup barrier of the copy pipeline.  __syncthreads() (and even an
+// LDS-only fence, since copies into LDS count as vector memory operations)
+// waits for every outstanding global->LDS copy (vmcnt(0)) and would drain
+// the pipeline at each chunk.  Here each wave has already waited for its own
+// copies of the chunk about to be read (WAIT_VMCNT); this waits for its LDS
+// reads (lgkmcnt(0): nobody overwrites a stage still being read), and the
+// empty asm statements keep the compiler from moving memory accesses across
+// the s_barrier.
+__device__ __attribute__((always_inline)) inline void lds_barrier() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4)); // lgkmcnt(0), vmcnt/expcnt untouched
+  __asm__ __volatile__("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  __asm__ __volatile__("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
 }
+
+// Scheduling hint for one k step: spread its NDMA global->LDS copies and NRD
+// fragment reads evenly over its NMF MFMAs (the compiler otherwise bunches
+// the reads after the last MFMA, and the next k step waits for them).
+// Masks: 0x008 MFMA, 0x020 vector-memory read, 0x100 LDS read.
+template <int NDMA, int NRD, int NMF> struct Interleave {
+  static constexpr int NMEM = NDMA + NRD;
+  static constexpr int GAP = NMEM > 0 ? (NMF / (NMEM + 1) > 0 ? NMF / (NMEM + 1) : 1) : NMF;
+  template <int I> __device__ static __attribute__((always_inline)) inline void step() {
+    if constexpr (I < NMEM) {
+      if constexpr (I < NDMA)
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      else
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      if constexpr ((I + 1) * GAP <= NMF) __builtin_amdgcn_sched_group_barrier(0x008, GAP, 0);
+      step<I + 1>();
+    } else if constexpr (NMF > NMEM * GAP) {
+      __builtin_amdgcn_sched_group_barrier(0x008, NMF - NMEM * GAP, 0);
+    }
+  }
+  __device__ static __attribute__((always_inline)) inline void apply() { step<0>(); }
+};
 
 template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int NS>
 struct TileShape {
@@ -470,7 +509,7 @@ struct TileShape {
 
 // One BM x BN output tile of one problem: k-chunks [c0, c1) of slice `slice`
 // (the range is computed here), accumulated in registers and stored.
-template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int VAR>
 __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &g, const int bm, const int bn,
                                                                 const int slice, u32x4 *lds) {
   constexpr int T = MF::T, KF = MF::KF, FR = MF::FR;
@@ -483,7 +522,7 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   constexpr int LA = A_FR / NW, LB = B_FR / NW, LPW = LA + LB;
   static_assert(A_FR % NW == 0 && B_FR % NW == 0, "fragments must split evenly over waves");
   static_assert(KFC % FR == 0 && TM >= 1 && TN >= 1, "bad tile");
-  static_assert(NS >= 2 && LPW * (NS - 2) <= 63, "too many loads in flight for vmcnt");
+  static_assert(NS >= 2 && LPW * (NS - 1) <= 63, "too many loads in flight for vmcnt");
   using Acc = typename MF::Acc;
   using Raw = typename MF::Raw;
 
@@ -563,7 +602,17 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
           bfr[buf][p][j][r] = base[(A_FR + (p * RB + wn * TN + j) * KFC + kf + r) * 64 + lane];
   };
   auto mfmas = [&](int buf) __attribute__((always_inline)) {
-    if constexpr (MODE == MODE_SPLIT3) {
+    if constexpr (MODE == MODE_SPLIT3 && (VAR & 1)) {
+      // product-major: consecutive MFMAs go to different accumulators; each
+      // accumulator still adds lo*hi, hi*lo, hi*hi in that order
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            MF::mma(acc[i][j], af[buf][q == 0 ? 1 : 0][i], bfr[buf][q == 1 ? 1 : 0][j]);
+    } else if constexpr (MODE == MODE_SPLIT3) {
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -580,29 +629,75 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     }
   };
 
-  // prologue: NS-1 chunks in flight
+  // The fragment reads run one k step ahead of the MFMAs, across chunk
+  // boundaries too.  During the last k step of chunk c the wave waits for
+  // its copies of chunk c+1 and meets the others at the barrier; after it,
+  // chunk c+1 is visible to all and chunk c's stage is read by nobody (its
+  // last fragments are already in registers), so that stage is refilled with
+  // chunk c+NS and chunk c+1's first fragments are read while chunk c's last
+  // MFMAs run.  Copies past the last chunk are clamped to it (into stages
+  // nobody reads again), so NS-1 chunks are always in flight behind the one
+  // being waited for, every wait takes a constant count and the steady-state
+  // loop has no branches.
+  constexpr int NKS = KFC / FR;
+  constexpr int NRD = P * (TM + TN) * FR;                                 // LDS reads per k step
+  constexpr int NMF = TM * TN * (MODE == MODE_SPLIT3 ? 3 : 1) * MF::NMMA; // MFMAs per k step
+  if (c0 < c1) {
+    const int clast = c1 - 1;
+    int stage = 0;
+    auto body = [&](auto b0, int c, auto is_last) __attribute__((always_inline)) {
+      constexpr int B0 = decltype(b0)::value;
+      const int next = stage == NS - 1 ? 0 : stage + 1;
 #pragma unroll
-  for (int u = 0; u < NS - 1; u++)
-    if (c0 + u < c1) issue(c0 + u, u);
-
-  int stage = 0; // stage holding chunk c
-  for (int c = c0; c < c1; c++) {
-    // this wave's copies of chunk c are done once at most the later chunks'
-    // copies are outstanding; the barrier makes every wave's copies visible
-    wait_vmcnt_upto<LPW * (NS - 2)>(LPW * min(NS - 2, c1 - 1 - c));
-    __syncthreads();
-    // the stage before this one held chunk c-1, which every wave has finished
-    if (c + NS - 1 < c1) issue(c + NS - 1, stage == 0 ? NS - 1 : stage - 1);
-    fload(stage, 0, 0);
+      for (int ks = 0; ks < NKS; ks++) {
+        const int buf = (B0 + ks) & 1;
+        if (ks + 1 < NKS) {
+          fload(stage, (ks + 1) * FR, buf ^ 1);
+          mfmas(buf);
+          Interleave<0, NRD, NMF>::apply();
+        } else if constexpr (!decltype(is_last)::value) {
+          __builtin_amdgcn_sched_barrier(0);
+          WAIT_VMCNT(LPW * (NS - 2));
+          lds_barrier();
+          issue(min(c + NS, clast), stage);
+          fload(next, 0, buf ^ 1);
+          mfmas(buf);
+          Interleave<LPW, NRD, NMF>::apply();
+        } else {
+          mfmas(buf);
+        }
+      }
+      stage = next;
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using More = std::false_type;
+    using Last = std::true_type;
 #pragma unroll
-    for (int kf = 0; kf < KFC; kf += FR) {
-      const int buf = (kf / FR) & 1;
-      if (kf + FR < KFC) fload(stage, kf + FR, buf ^ 1);
-      if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
-      mfmas(buf);
-      if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+    for (int u = 0; u < NS; u++) issue(min(c0 + u, clast), u);
+    WAIT_VMCNT(LPW * (NS - 1));
+    lds_barrier();
+    fload(0, 0, 0);
+    int c = c0;
+    if constexpr (NKS % 2 == 0) {
+      for (; c < clast; c++) body(I0(), c, More());
+      body(I0(), c, Last());
+    } else {
+      // the register buffer alternates per chunk: two chunks per iteration
+      for (; c + 1 < clast; c += 2) {
+        body(I0(), c, More());
+        body(I1(), c + 1, More());
+      }
+      if (c < clast) {
+        body(I0(), c, More());
+        body(I1(), c + 1, Last());
+      } else {
+        body(I0(), c, Last());
+      }
     }
-    stage = stage == NS - 1 ? 0 : stage + 1;
+    // the clamped copies past the last chunk may still be in flight: let
+    // them land before the stages are reused or the workgroup ends
+    WAIT_VMCNT(0);
   }
 
   // epilogue
@@ -623,7 +718,7 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
       }
 }
 
-template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int VAR>
 __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   const GemmArgs g = gin; // local copy: the tile code captures it by reference
   // NS stages of fragments, filled straight from HBM by global_load_lds
@@ -638,7 +733,7 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   const int gsz = min(g.nbm - first, 8);
   const int bm = first + (pid % per_group) % gsz;
   const int bn = (pid % per_group) / gsz;
-  gemm_tile<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, PRIO>(g, bm, bn, blockIdx.y, lds);
+  gemm_tile<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>(g, bm, bn, blockIdx.y, lds);
 }
 
 // split-K: sum the slices in slice order, then the final epilogue
@@ -786,13 +881,13 @@ int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix
 // configuration tables
 // ===========================================================================
 
-template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int PRIO>
+template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int VAR>
 static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, PRIO>), grid, dim3(WGM * WGN * 64), 0, s, g);
+  hipLaunchKernelGGL((k_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>), grid, dim3(WGM * WGN * 64), 0, s, g);
 }
 
-#define CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, PRIO) \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, PRIO>}
+#define CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, VAR) \
+  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>}
 #define CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS) CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, 0)
 #define CFG(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT) CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, 2)
 
@@ -808,10 +903,11 @@ static const GemmConfig kF32[] = {
     CFGS(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 8, 4),
 };
 static const GemmConfig kSplit[] = {
-    CFGS(MfBF16, 256, 256, 16, 4, 2, MODE_SPLIT3, EPI_F32, 1, 4),
+    CFGP(MfBF16, 256, 256, 32, 4, 2, MODE_SPLIT3, EPI_F32, 1, 2, 1),
     CFGS(MfBF16, 256, 256, 32, 2, 4, MODE_SPLIT3, EPI_F32, 1, 2),
     CFGS(MfBF16, 256, 256, 16, 2, 2, MODE_SPLIT3, EPI_F32, 1, 4),
-    CFGS(MfBF16, 256, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 3),
+    CFGS(MfBF16, 256, 128, 32, 4, 2, MODE_SPLIT3, EPI_F32, 1, 3),
+    CFGS(MfBF16, 128, 256, 32, 2, 4, MODE_SPLIT3, EPI_F32, 1, 3),
     CFGS(MfBF16, 128, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 4),
     CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 4),
     CFGS(MfBF16, 128, 128, 32, 2, 2, MODE_SPLIT3, EPI_F32, 4, 4),
@@ -819,6 +915,9 @@ static const GemmConfig kSplit[] = {
 };
 static const GemmConfig kFP8[] = {
     CFGS(MfFP8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
+    CFGS(MfFP8, 256, 256, 64, 4, 2, MODE_SINGLE, EPI_FP8, 1, 4),
+    CFGS(MfFP8, 256, 256, 128, 4, 2, MODE_SINGLE, EPI_FP8, 1, 2),
+    CFGS(MfFP8, 256, 256, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
     CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
     CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
     CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 4, 4),
@@ -828,12 +927,16 @@ static const GemmConfig kF64[] = {
     CFGS(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
     CFGS(MfF64, 128, 64, 16, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
     CFGS(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 8, 4),
+    CFGS(MfF64, 256, 128, 16, 4, 2, MODE_SINGLE, EPI_F64, 1, 3),
+    CFGS(MfF64, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F64, 1, 2),
 };
 static const GemmConfig kI8[] = {
     CFGS(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
     CFGS(MfI8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
     CFGS(MfI8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
     CFGS(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 4, 4),
+    CFGS(MfI8, 256, 256, 64, 4, 2, MODE_SINGLE, EPI_INT, 1, 4),
+    CFGS(MfI8, 256, 256, 128, 4, 2, MODE_SINGLE, EPI_INT, 1, 2),
 };
 #undef CFG
 
@@ -899,12 +1002,134 @@ static int digit_planes(NumcDType out) {
   return s >= 4 ? 4 : (int)s;
 }
 
+// Signed base-256 digits of the 16 values of one fragment lane into P planes
+// (16 bytes per plane), and the planes' nonzero mask for the wave.
+template <class S, int P>
+__device__ __attribute__((always_inline)) inline void digits_unit(const PackArgs &a, const uint32_t *t, long long off,
+                                                                  bool live) {
+  uint32_t w[P][4];
+#pragma unroll
+  for (int p = 0; p < P; p++) w[p][0] = w[p][1] = w[p][2] = w[p][3] = 0;
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      unsigned long long u = (unsigned long long)(long long)(S)t[e];
+#pragma unroll
+      for (int p = 0; p < P; p++) {
+        int8_t d = (int8_t)(u & 0xFFu);
+        u = (u - (unsigned long long)(long long)d) >> 8;
+        w[p][e >> 2] |= (uint32_t)(uint8_t)d << (8 * (e & 3));
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++)
+      *(uint4 *)(a.dst + p * a.plane_bytes + off) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+  }
+  int mask = 0;
+#pragma unroll
+  for (int p = 0; p < P; p++)
+    if (__ballot((w[p][0] | w[p][1] | w[p][2] | w[p][3]) != 0)) mask |= 1 << p;
+  if ((threadIdx.x & 63) == 0 && mask) flags_or(a.flags, mask);
+}
+
+// one 16-byte chunk of source values (EPC of them) into LDS as raw 32-bit
+// words; (S)word gives the value back
+template <class S>
+__device__ __attribute__((always_inline)) inline void load_chunk(const S *src, long long n_ok, uint32_t *v) {
+  constexpr int EPC = 16 / sizeof(S);
+  if (n_ok >= EPC && ((((uintptr_t)src) & 15) == 0)) {
+    union { uint4 q; S e[EPC]; } u;
+    u.q = *(const uint4 *)src;
+#pragma unroll
+    for (int e = 0; e < EPC; e++) v[e] = (uint32_t)u.e[e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < EPC; e++) v[e] = e < n_ok ? (uint32_t)src[e] : 0u;
+  }
+}
+
+// integer digit planes, rows with unit k-stride: 32 rows x 128 k tiles
+// staged through LDS so that global reads are 16-byte and coalesced
+template <class S, int P>
+__global__ void __launch_bounds__(256) k_pack_digits_tiled(PackArgs a) {
+  constexpr int TK = 128, PITCH = TK + 1, EPC = 16 / sizeof(S), CPR = TK / EPC;
+  __shared__ uint32_t tile[32 * PITCH];
+  const int nkt = (a.nKF * 32 + TK - 1) / TK;
+  const long long rt = blockIdx.x / nkt;
+  const int kt = blockIdx.x % nkt;
+  const int tid = threadIdx.x;
+  const long long row0 = rt * 32;
+  const int k0 = kt * TK;
+  for (int idx = tid; idx < 32 * CPR; idx += 256) {
+    const int r = idx / CPR, c = (idx % CPR) * EPC;
+    const long long row = row0 + r;
+    uint32_t v[EPC];
+    const long long n_ok = row < a.R ? (long long)a.K - (k0 + c) : 0;
+    load_chunk<S>((const S *)a.src + (row < a.R ? row * a.rs + k0 + c : 0), n_ok, v);
+#pragma unroll
+    for (int e = 0; e < EPC; e++) tile[r * PITCH + c + e] = v[e];
+  }
+  __syncthreads();
+  // 4 fragments x 64 lanes: one unit per thread
+  const int f = tid >> 6, lane = tid & 63;
+  const int kf = kt * (TK / 32) + f;
+  uint32_t t[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) t[e] = tile[(lane & 31) * PITCH + f * 32 + (lane >> 5) * 16 + e];
+  digits_unit<S, P>(a, t, (rt * a.nKF + kf) * 1024 + lane * 16, kf < a.nKF);
+}
+
+// integer digit planes, rows contiguous (unit row stride): 128 rows x 32 k
+template <class S, int P>
+__global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
+  constexpr int TR = 128, TK = 32, PITCH = TK + 1, EPC = 16 / sizeof(S), CPK = TR / EPC;
+  __shared__ uint32_t tile[TR * PITCH];
+  // one k fragment (32 k) per tile; consecutive blocks take neighbouring
+  // row blocks at one k range, so the segments they read complete each
+  // other's source rows
+  const int nrb = (a.nRT + 3) / 4;
+  const long long rb = blockIdx.x % nrb;
+  const int kt = blockIdx.x / nrb;
+  const int tid = threadIdx.x;
+  const long long row0 = rb * TR;
+  const int k0 = kt * TK;
+  for (int idx = tid; idx < TK * CPK; idx += 256) {
+    const int kk = idx / CPK, r = (idx % CPK) * EPC;
+    const int k = k0 + kk;
+    uint32_t v[EPC];
+    const long long n_ok = k < a.K ? (long long)a.R - (row0 + r) : 0;
+    load_chunk<S>((const S *)a.src + (k < a.K ? (long long)k * a.ks + row0 + r : 0), n_ok, v);
+#pragma unroll
+    for (int e = 0; e < EPC; e++) tile[(r + e) * PITCH + kk] = v[e];
+  }
+  __syncthreads();
+  // 4 row fragments x 64 lanes: one unit per thread
+  const int rl = tid >> 6, lane = tid & 63;
+  const long long rt = rb * 4 + rl;
+  uint32_t t[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) t[e] = tile[(rl * 32 + (lane & 31)) * PITCH + (lane >> 5) * 16 + e];
+  digits_unit<S, P>(a, t, (rt * a.nKF + kt) * 1024 + lane * 16, rt < a.nRT);
+}
+
+template <class S, int P> static void launch_digits_p(PackArgs &pk, unsigned nb, hipStream_t st) {
+  if (pk.ks == 1) {
+    dim3 grid((unsigned)((long long)pk.nRT * cdiv((long long)pk.nKF * 32, 128)));
+    hipLaunchKernelGGL((k_pack_digits_tiled<S, P>), grid, dim3(256), 0, st, pk);
+  } else if (pk.rs == 1) {
+    dim3 grid((unsigned)(cdiv(pk.nRT, 4) * pk.nKF));
+    hipLaunchKernelGGL((k_pack_digits_tiled_rows<S, P>), grid, dim3(256), 0, st, pk);
+  } else {
+    hipLaunchKernelGGL((k_pack_digits<S, P>), dim3(nb), dim3(256), 0, st, pk);
+  }
+}
+
 template <class S>
 static void launch_digits(int planes, PackArgs &pk, unsigned nb, hipStream_t st) {
   switch (planes) {
-  case 1: hipLaunchKernelGGL((k_pack_digits<S, 1>), dim3(nb), dim3(256), 0, st, pk); break;
-  case 2: hipLaunchKernelGGL((k_pack_digits<S, 2>), dim3(nb), dim3(256), 0, st, pk); break;
-  default: hipLaunchKernelGGL((k_pack_digits<S, 4>), dim3(nb), dim3(256), 0, st, pk); break;
+  case 1: launch_digits_p<S, 1>(pk, nb, st); break;
+  case 2: launch_digits_p<S, 2>(pk, nb, st); break;
+  default: launch_digits_p<S, 4>(pk, nb, st); break;
   }
 }
 

@@ -28,11 +28,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // MFMA flavours.  T: rows per fragment; KF: k per fragment; FR: fragments of
-// each operand consumed by one mma() call.
+// each operand consumed by one mma() call; NMMA: MFMA instructions per call.
 // ---------------------------------------------------------------------------
 
 struct MfF32 { // v_mfma_f32_32x32x2_f32 x4 per fragment
-  static constexpr int T = 32, KF = 8, FR = 1, NACC = 16;
+  static constexpr int T = 32, KF = 8, FR = 1, NACC = 16, NMMA = 4;
   using Acc = f32x16;
   using Raw = float;
   __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
@@ -46,7 +46,7 @@ struct MfF32 { // v_mfma_f32_32x32x2_f32 x4 per fragment
 };
 
 struct MfBF16 { // v_mfma_f32_32x32x16_bf16
-  static constexpr int T = 32, KF = 16, FR = 1, NACC = 16;
+  static constexpr int T = 32, KF = 16, FR = 1, NACC = 16, NMMA = 1;
   using Acc = f32x16;
   using Raw = float;
   __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
@@ -59,7 +59,7 @@ struct MfBF16 { // v_mfma_f32_32x32x16_bf16
 };
 
 struct MfI8 { // v_mfma_i32_32x32x32_i8
-  static constexpr int T = 32, KF = 32, FR = 1, NACC = 16;
+  static constexpr int T = 32, KF = 32, FR = 1, NACC = 16, NMMA = 1;
   using Acc = i32x16;
   using Raw = int;
   __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
@@ -72,7 +72,7 @@ struct MfI8 { // v_mfma_i32_32x32x32_i8
 };
 
 struct MfFP8 { // v_mfma_scale_f32_32x32x64_f8f6f4, both operands OCP e4m3, unit scales
-  static constexpr int T = 32, KF = 32, FR = 2, NACC = 16;
+  static constexpr int T = 32, KF = 32, FR = 2, NACC = 16, NMMA = 1;
   using Acc = f32x16;
   using Raw = float;
   __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
@@ -85,7 +85,7 @@ struct MfFP8 { // v_mfma_scale_f32_32x32x64_f8f6f4, both operands OCP e4m3, unit
 };
 
 struct MfF64 { // v_mfma_f64_16x16x4_f64 x2 per fragment
-  static constexpr int T = 16, KF = 8, FR = 1, NACC = 4;
+  static constexpr int T = 16, KF = 8, FR = 1, NACC = 4, NMMA = 2;
   using Acc = f64x4;
   using Raw = double;
   __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {

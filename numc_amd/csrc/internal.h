@@ -49,6 +49,7 @@ struct NumcCtx {
   bool events_made;
   Workspace ws;      // packed operands, partial sums
   Workspace ws_aux;  // flags, scales, batched tables
+  Workspace ws_stage; // host operands of numc_amd_gemm
   void *batch_cache; // BatchCache (batched.hip)
   void *prof;        // Profiler (runtime.cpp), null unless profiling is on
   void *tune;        // autotuner scratch operands (autotune.hip)

@@ -44,9 +44,9 @@ static int check_matmul(const NumcArray *a, const NumcArray *b, const NumcArray 
 }
 
 // which kernel family runs (or -1 for the 64-bit integer kernel)
-static int pick_family(const NumcArray *a, const NumcMatmulOpts *o) {
+static int pick_family_dt(NumcDType dt, const NumcMatmulOpts *o) {
   int method = o ? o->method : NUMC_METHOD_AUTO;
-  switch (a->dtype) {
+  switch (dt) {
   case NUMC_DTYPE_FLOAT32:
     if (method == NUMC_METHOD_SPLIT_BF16) return NUMC_FAMILY_SPLIT_BF16;
     if (method == NUMC_METHOD_FP8_SCALED) return NUMC_FAMILY_FP8;
@@ -58,11 +58,13 @@ static int pick_family(const NumcArray *a, const NumcMatmulOpts *o) {
   default:
     if (method != NUMC_METHOD_AUTO && method != NUMC_METHOD_PLAIN)
       return NUMC_SET_ERROR(NUMC_ERR_TYPE, "matmul: method %d needs float32 operands", method);
-    if (a->dtype == NUMC_DTYPE_FLOAT64) return NUMC_FAMILY_F64;
-    if (a->dtype == NUMC_DTYPE_INT64 || a->dtype == NUMC_DTYPE_UINT64) return 100;
+    if (dt == NUMC_DTYPE_FLOAT64) return NUMC_FAMILY_F64;
+    if (dt == NUMC_DTYPE_INT64 || dt == NUMC_DTYPE_UINT64) return 100;
     return NUMC_FAMILY_I8;
   }
 }
+
+static int pick_family(const NumcArray *a, const NumcMatmulOpts *o) { return pick_family_dt(a->dtype, o); }
 
 // ---- residency staging -----------------------------------------------------
 
@@ -136,9 +138,31 @@ int numc_select_config(int family, int M, int N, int K, int flags) {
   return heuristic_config(family, M, N, K);
 }
 
+extern "C" NUMC_API int numc_selector_choice(int family, int M, int N, int K, int flags) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "bad family %d", family);
+  return numc_select_config(family, M, N, K, flags);
+}
+
 // ---- entry points ---------------------------------------------------------------
 
 static inline long long es_stride(size_t bytes, size_t es) { return (long long)(bytes / es); }
+
+// configuration choice, workspaces and launch of one problem whose operands
+// are already on the device
+static int launch_problem(NumcCtx *ctx, const GemmProblem &p, const NumcMatmulOpts *opts) {
+  if (p.family == 100) return numc_i64_launch(ctx, p);
+  // layout flags as the autotuner measures them: A stored K x M, B stored N x K, padded rows
+  const int lay = (p.csa != 1 ? 1 : 0) | (p.csb != 1 ? 2 : 0) |
+                  ((p.csa == 1 && p.rsa > p.K) || (p.csb == 1 && p.rsb > p.N) ? 4 : 0);
+  int cfg_idx = opts && opts->config >= 0 ? opts->config : numc_select_config(p.family, p.M, p.N, p.K, lay);
+  const GemmConfig *cfg = numc_gemm_config(p.family, cfg_idx);
+  if (!cfg) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: no config %d for family %d", cfg_idx, p.family);
+  size_t aux = 0, need = numc_gemm_workspace(p, *cfg, &aux);
+  int err;
+  if ((err = numc_ws_reserve(ctx, &ctx->ws, need))) return err;
+  if ((err = numc_ws_reserve(ctx, &ctx->ws_aux, aux))) return err;
+  return numc_gemm_launch(ctx, p, *cfg, ctx->ws.ptr, ctx->ws_aux.ptr);
+}
 
 int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts,
                     bool finish) {
@@ -170,21 +194,7 @@ int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, cons
   if ((err = stage_in(ctx, b, &p.B))) return err;
   if ((err = stage_out_begin(ctx, out, &p.C))) return err;
 
-  if (family == 100) {
-    err = numc_i64_launch(ctx, p);
-  } else {
-    // layout flags as the autotuner measures them: A stored K x M, B stored N x K, padded rows
-    const int lay = (p.csa != 1 ? 1 : 0) | (p.csb != 1 ? 2 : 0) |
-                    ((p.csa == 1 && p.rsa > p.K) || (p.csb == 1 && p.rsb > p.N) ? 4 : 0);
-    int cfg_idx = opts && opts->config >= 0 ? opts->config : numc_select_config(family, p.M, p.N, p.K, lay);
-    const GemmConfig *cfg = numc_gemm_config(family, cfg_idx);
-    if (!cfg) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: no config %d for family %d", cfg_idx, family);
-    size_t aux = 0, need = numc_gemm_workspace(p, *cfg, &aux);
-    if ((err = numc_ws_reserve(ctx, &ctx->ws, need))) return err;
-    if ((err = numc_ws_reserve(ctx, &ctx->ws_aux, aux))) return err;
-    err = numc_gemm_launch(ctx, p, *cfg, ctx->ws.ptr, ctx->ws_aux.ptr);
-  }
-  if (err) return err;
+  if ((err = launch_problem(ctx, p, opts))) return err;
   if ((err = stage_out_end(ctx, out, p.C))) return err;
   return finish ? numc_finish(ctx) : 0;
 }
@@ -238,5 +248,81 @@ extern "C" NUMC_API int numc_matmul_naive(const NumcArray *a, const NumcArray *b
   if ((err = numc_naive_launch(ctx, a->dtype, da, db, dc, M, K, N))) return err;
   if (out->st->residency == RES_HOST && out->size)
     HIP_TRY(hipMemcpyAsync(out->data, dc, out->size * out->elem_size, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  return numc_finish(ctx);
+}
+
+// ---- pointer-level entry point ----------------------------------------------------
+//
+// For a host library that keeps its own arrays: numc's numc_matmul
+// (src/matmul/matmul.c:347-389) would call this after its own _check_matmul
+// with its arrays' data pointers, shapes and byte strides.
+
+static size_t span_bytes(size_t rows, size_t cols, size_t rs, size_t cs, size_t es) {
+  if (rows == 0 || cols == 0) return 0;
+  return (rows - 1) * rs + (cols - 1) * cs + es;
+}
+
+extern "C" NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, size_t M, size_t N, size_t K,
+                                      const void *a, size_t rsa, size_t csa, const void *b, size_t rsb,
+                                      size_t csb, void *c, size_t rso, size_t cso, const NumcMatmulOpts *opts) {
+  if (!ctx || (M && K && !a) || (K && N && !b) || (M && N && !c))
+    return NUMC_SET_ERROR(NUMC_ERR_NULL, "amd_gemm: NULL pointer (ctx=%p a=%p b=%p c=%p)", (void *)ctx, a, b, c);
+  const bool widen = opts && (opts->flags & NUMC_OPT_WIDEN);
+  bool types_ok = dtype >= 0 && dtype <= NUMC_DTYPE_FLOAT64 && (dtype == out_dtype);
+  if (!types_ok && widen)
+    types_ok = (dtype == NUMC_DTYPE_INT8 || dtype == NUMC_DTYPE_UINT8) &&
+               (out_dtype == NUMC_DTYPE_INT32 || out_dtype == NUMC_DTYPE_UINT32);
+  if (!types_ok) return NUMC_SET_ERROR(NUMC_ERR_TYPE, "amd_gemm: dtype mismatch (in=%d out=%d)", dtype, out_dtype);
+  const size_t lim = (size_t)1 << 30;
+  if (M >= lim || N >= lim || K >= lim) return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "amd_gemm: dimensions must be below 2^30");
+  const size_t ein = numc_dtype_size((NumcDType)dtype), eout = numc_dtype_size((NumcDType)out_dtype);
+  if (rsa % ein || csa % ein || rsb % ein || csb % ein || rso % eout || cso % eout)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "amd_gemm: strides must be multiples of the element size");
+  int family = pick_family_dt((NumcDType)dtype, opts);
+  if (family < 0) return family;
+  if (M == 0 || N == 0) return 0;
+  int err;
+  if ((err = numc_gpu_init(ctx))) return err;
+
+  GemmProblem p{};
+  p.family = family;
+  p.in_dt = (NumcDType)dtype;
+  p.out_dt = (NumcDType)out_dtype;
+  p.M = (int)M;
+  p.N = (int)N;
+  p.K = (int)K;
+  p.rsa = es_stride(rsa, ein);
+  p.csa = es_stride(csa, ein);
+  p.rsb = es_stride(rsb, ein);
+  p.csb = es_stride(csb, ein);
+  p.rso = es_stride(rso, eout);
+  p.cso = es_stride(cso, eout);
+  const bool device = opts && (opts->flags & NUMC_OPT_DEVICE_POINTERS);
+  if (device) {
+    p.A = a;
+    p.B = b;
+    p.C = c;
+  } else {
+    // host operands: stage the spans the strided views cover
+    const size_t sa = span_bytes(M, K, rsa, csa, ein), sb = span_bytes(K, N, rsb, csb, ein);
+    const size_t sc = span_bytes(M, N, rso, cso, eout);
+    const size_t oa = 0, ob = (sa + 255) & ~(size_t)255, oc = ob + ((sb + 255) & ~(size_t)255);
+    if ((err = numc_ws_reserve(ctx, &ctx->ws_stage, oc + sc))) return err;
+    unsigned char *d = (unsigned char *)ctx->ws_stage.ptr;
+    if (sa) HIP_TRY(hipMemcpyAsync(d + oa, a, sa, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy H2D");
+    if (sb) HIP_TRY(hipMemcpyAsync(d + ob, b, sb, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy H2D");
+    // a view with gaps: bring the gaps over so the download keeps them
+    if (sc > M * N * eout) HIP_TRY(hipMemcpyAsync(d + oc, c, sc, hipMemcpyHostToDevice, ctx->stream), "hipMemcpy H2D");
+    p.A = d + oa;
+    p.B = d + ob;
+    p.C = d + oc;
+  }
+  if ((err = launch_problem(ctx, p, opts))) return err;
+  if (!device) {
+    const size_t sc = span_bytes(M, N, rso, cso, eout);
+    HIP_TRY(hipMemcpyAsync(c, p.C, sc, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy D2H");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return 0;
+  }
   return numc_finish(ctx);
 }

@@ -276,6 +276,7 @@ extern "C" NUMC_API void numc_ctx_free(NumcCtx *ctx) {
   }
   if (ctx->ws.ptr) (void)hipFree(ctx->ws.ptr);
   if (ctx->ws_aux.ptr) (void)hipFree(ctx->ws_aux.ptr);
+  if (ctx->ws_stage.ptr) (void)hipFree(ctx->ws_stage.ptr);
   if (ctx->events_made)
     for (auto &e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->gpu_ready) (void)hipStreamDestroy(ctx->stream);

@@ -176,3 +176,31 @@ def test_config_tables():
         for i in range(n):
             c = nc.config_get(fam, i)
             assert c["bm"] % 16 == 0 and c["bn"] % 16 == 0 and c["split_k"] >= 1
+
+
+def test_pointer_entry_error_contract(ctx):
+    """numc_amd_gemm validates before touching the GPU: NULL -> dtype -> shape -> strides."""
+    L = nc.lib()
+    a = np.ones((4, 3), np.float32)
+    b = np.ones((3, 5), np.float32)
+    c = np.zeros((4, 5), np.float32)
+    opts = nc.MatmulOpts(0, -1, 0)
+
+    def call(h, dt, odt, M, N, K, pa, pb, pc, rsa=12, csa=4, rsb=20, csb=4, rso=20, cso=4, o=opts):
+        return L.numc_amd_gemm(h, dt, odt, M, N, K, pa, rsa, csa, pb, rsb, csb, pc, rso, cso, C.byref(o))
+
+    pa, pb, pc = a.ctypes.data, b.ctypes.data, c.ctypes.data
+    assert call(None, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, pa, pb, pc) == nc.ERR_NULL
+    assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, None, pb, pc) == nc.ERR_NULL
+    assert call(ctx.h, nc.FLOAT32, nc.FLOAT64, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE
+    assert call(ctx.h, 10, 10, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE
+    assert call(ctx.h, nc.INT8, nc.INT32, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE   # widening needs the flag
+    assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 1 << 30, 5, 3, pa, pb, pc) == nc.ERR_SHAPE
+    assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, pa, pb, pc, csa=2) == nc.ERR_VALUE
+    bad = nc.MatmulOpts(nc.METHOD_SPLIT_BF16, -1, 0)
+    assert call(ctx.h, nc.FLOAT64, nc.FLOAT64, 4, 5, 3, pa, pb, pc, rsa=24, csa=8, rsb=40, csb=8, rso=40, cso=8,
+                o=bad) == nc.ERR_TYPE
+    assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 0, 5, 3, pa, pb, pc) == 0      # empty: nothing to do
+    if not os.path.exists("/dev/kfd"):
+        assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, pa, pb, pc) == nc.ERR_INTERNAL
+        assert np.array_equal(c, np.zeros((4, 5), np.float32)), "no CPU fallback"

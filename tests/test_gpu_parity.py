@@ -7,6 +7,7 @@ The cases mirror numc's tests/matmul/test_matmul_naive.c and test_matmul.c
 (fixture tests/golden/numc_known_answers.json) and add random, strided,
 empty and full-range cases plus every kernel configuration of every family.
 """
+import ctypes as C
 import json
 import os
 
@@ -151,8 +152,23 @@ def test_fp8_configs(ctx, cfg):
         ref, absr = oracle.fref(qa, qb)   # exact operands of the FP8 products
         err = oracle.err_metrics(got, ref, absr)[0]
         # the f8f6f4 MFMA sums its 64 products internally with less than
-        # binary32 precision (measured ~2^-17 relative per instruction)
-        assert err <= -(-K // 64) * 2.0 ** -15 + (K + 2) * U, (cfg, M, K, N, err)
+        # binary32 precision: measured up to 2^-14.3 per instruction for
+        # N(0,1) operands and 2^-11.5 when a row's magnitudes spread over 2^12
+        assert err <= -(-K // 64) * 2.0 ** -11 + (K + 2) * U, (cfg, M, K, N, err)
+
+
+def test_fp8_spread_magnitudes(ctx):
+    """Operands whose magnitudes spread over 2^12 within each row and column:
+    the case where the f8f6f4 MFMA's internal sum loses the most."""
+    rng = np.random.default_rng(310)
+    for M, K, N in [(256, 31, 256), (256, 64, 256), (128, 1000, 192)]:
+        a = (rng.uniform(-1, 1, (M, K)) * 2.0 ** rng.integers(-12, 0, (M, K))).astype(np.float32)
+        b = (rng.uniform(-1, 1, (K, N)) * 2.0 ** rng.integers(-12, 0, (K, N))).astype(np.float32)
+        got = gemm(ctx, a, b, method=nc.METHOD_FP8_SCALED)
+        qa, qb = oracle.fp8_quantized_inputs(a, b)
+        ref, absr = oracle.fref(qa, qb)
+        err = oracle.err_metrics(got, ref, absr)[0]
+        assert err <= -(-K // 64) * 2.0 ** -11 + (K + 2) * U, (M, K, N, err)
 
 
 @pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F64)))
@@ -243,6 +259,51 @@ def test_strided_views(ctx):
         mask = np.ones(full.shape, bool)
         mask[:, 2:62:2] = False
         assert np.all(full[mask] == 7), "bytes between output columns must be left alone"
+
+
+def test_pointer_entry_host_and_device(ctx):
+    """numc_amd_gemm (pointer-level entry for a host library keeping its own
+    arrays) on host pointers with strided views, widening, and device pointers."""
+    rng = np.random.default_rng(21)
+    # float32, A transposed view, B every other column, C with gaps between columns
+    base_a = rng.standard_normal((90, 70)).astype(np.float32)
+    base_b = rng.standard_normal((80, 61)).astype(np.float32)
+    a, b = base_a.T[:, :80], base_b[:, 1:61:2]
+    full = np.full((70, 65), 7, np.float32)
+    c = full[:, 2:62:2]
+    for method, bound in [(nc.METHOD_PLAIN, 100 * U), (nc.METHOD_SPLIT_BF16, 3 * 2.0 ** -16 + 82 * U)]:
+        full[:] = 7
+        nc.check(nc.gemm_pointers(ctx, a, b, c, method=method), "numc_amd_gemm f32")
+        assert cw_err(c.copy(), np.ascontiguousarray(a), np.ascontiguousarray(b)) <= bound
+        mask = np.ones(full.shape, bool)
+        mask[:, 2:62:2] = False
+        assert np.all(full[mask] == 7), "bytes between output columns must be left alone"
+    # every integer dtype up to 32 bits, exact wraparound; int8 -> int32 widening
+    for dt in [np.int8, np.uint16, np.int32]:
+        info = np.iinfo(dt)
+        x = rng.integers(info.min, info.max, (33, 47), endpoint=True).astype(dt)
+        y = rng.integers(info.min, info.max, (47, 29), endpoint=True).astype(dt)
+        z = np.zeros((33, 29), dt)
+        nc.check(nc.gemm_pointers(ctx, x, y, z), "numc_amd_gemm int")
+        assert np.array_equal(z, oracle.matmul_int(x, y))
+    x = rng.integers(-128, 127, (64, 300), endpoint=True).astype(np.int8)
+    y = rng.integers(-128, 127, (300, 50), endpoint=True).astype(np.int8)
+    z = np.zeros((64, 50), np.int32)
+    nc.check(nc.gemm_pointers(ctx, x, y, z, flags=nc.OPT_WIDEN), "numc_amd_gemm widen")
+    assert np.array_equal(z, x.astype(np.int64) @ y.astype(np.int64))
+    # device pointers: operands already in HBM (here: arrays made resident)
+    a2 = rng.uniform(-1, 1, (120, 96)).astype(np.float32)
+    b2 = rng.uniform(-1, 1, (96, 72)).astype(np.float32)
+    A, B = ctx.from_numpy(a2), ctx.from_numpy(b2)
+    O = ctx.array((120, 72), nc.FLOAT32, zeros=True)
+    for arr in (A, B, O):
+        arr.to_device()
+    L = nc.lib()
+    da, db, do = (L.numc_array_device_data(arr.h) for arr in (A, B, O))
+    opts = nc.MatmulOpts(nc.METHOD_PLAIN, -1, nc.OPT_DEVICE_POINTERS)
+    nc.check(L.numc_amd_gemm(ctx.h, nc.FLOAT32, nc.FLOAT32, 120, 72, 96, da, 96 * 4, 4, db, 72 * 4, 4,
+                             do, 72 * 4, 4, C.byref(opts)), "numc_amd_gemm device")
+    assert cw_err(O.numpy(), a2, b2) <= 98 * U
 
 
 def test_device_residency(ctx):
