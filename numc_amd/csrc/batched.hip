@@ -88,9 +88,12 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   const NumcDType in_dt = a[0]->dtype, out_dt = out[0]->dtype;
   for (size_t i = 1; i < count; i++)
     if (out[i]->dtype != out_dt) return NUMC_SET_ERROR(NUMC_ERR_TYPE, "matmul_batched: output dtypes differ");
+  bool same_in = true; // int8 and uint8 inputs may share a family when widening
+  for (size_t i = 1; i < count; i++) same_in = same_in && a[i]->dtype == in_dt;
   const bool batched_kernel =
       family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
-      (family == NUMC_FAMILY_I8 && in_dt == NUMC_DTYPE_INT8 && (out_dt == NUMC_DTYPE_INT8 || out_dt == NUMC_DTYPE_INT32));
+      (family == NUMC_FAMILY_I8 && same_in && in_dt == NUMC_DTYPE_INT8 &&
+       (out_dt == NUMC_DTYPE_INT8 || out_dt == NUMC_DTYPE_INT32));
   int err = numc_gpu_init(ctx);
   if (err) return err;
   if (!batched_kernel) { // other dtypes: one launch sequence per problem on the same stream
@@ -166,10 +169,11 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     da.nRT = p.nRTa; da.nKF = p.nKF; da.plane_bytes = p.pla;
     db.src = p.B; db.rs = p.csb; db.ks = p.rsb; db.R = p.N; db.K = p.K;
     db.nRT = p.nRTb; db.nKF = p.nKF; db.plane_bytes = p.plb;
+    const int ba = numc_batch_pack_blocks(family, da), bb = numc_batch_pack_blocks(family, db);
     descs.push_back(da);
-    prefix.push_back(prefix.back() + (long long)p.nRTa * p.nKF * 64);
+    prefix.push_back(prefix.back() + ba);
     descs.push_back(db);
-    prefix.push_back(prefix.back() + (long long)p.nRTb * p.nKF * 64);
+    prefix.push_back(prefix.back() + bb);
   }
   for (size_t oi = 0; oi < count; oi++) {
     const size_t i = order[oi];

@@ -38,6 +38,7 @@ struct PackArgs {
   long long plane_bytes;
   const int *row_exp; // FP8
   int *flags;         // integer digit planes: nonzero mask
+  int mode;           // batched pack: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 lane slots
 };
 
 // one problem of the persistent batched kernel (packed operands in a workspace)
@@ -51,6 +52,7 @@ struct BatchProblem {
 };
 
 int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc);
+int numc_batch_pack_blocks(int family, PackArgs &pk); // sets pk.mode
 int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
                       const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
                       hipStream_t st);

@@ -150,24 +150,6 @@ __global__ void k_pack_split_bf16(PackArgs a) {
   if (t < (long long)a.nRT * a.nKF * 64) pack_unit_split(a, t);
 }
 
-// many operands in one launch: unit t belongs to the descriptor whose unit
-// range [prefix[i], prefix[i+1]) holds it
-template <int FAM>
-__global__ void k_pack_batched(const PackArgs *descs, const long long *prefix, int ndesc) {
-  const long long t = blockIdx.x * 256LL + threadIdx.x;
-  if (t >= prefix[ndesc]) return;
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  const PackArgs a = descs[lo];
-  const long long u = t - prefix[lo];
-  if constexpr (FAM == NUMC_FAMILY_F32) pack_unit_f32(a, u);
-  else if constexpr (FAM == NUMC_FAMILY_SPLIT_BF16) pack_unit_split(a, u);
-  else pack_unit_i8(a, u);
-}
 
 // f32 * 2^row_exp[row] -> e4m3  (T=32, 16 per lane, KF=32)
 __global__ void k_pack_fp8(PackArgs a) {
@@ -193,13 +175,17 @@ __global__ void k_pack_fp8(PackArgs a) {
 // A block stages a 32-row x 128-k tile through LDS: reads are whole 512-byte
 // row segments (coalesced), writes are whole 1 KiB fragments.  MODE_PK: 0
 // plain f32 (KF 8), 1 bf16 hi/lo (KF 16), 2 e4m3 with row exponents (KF 32).
+// LDS words one pack tile needs (every tiled pack below fits)
+#define PACK_TILE_WORDS (128 * 33 + 64)
+
+// tile `bid` of a k-contiguous operand: 32 rows x 128 k through LDS
 template <int MODE_PK>
-__global__ void __launch_bounds__(256) k_pack_f32_tiled(PackArgs a) {
+__device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const PackArgs &a, long long bid, float *tile) {
   constexpr int TK = 128, PITCH = TK + 4;
-  __shared__ float tile[32 * PITCH];
+  static_assert(32 * PITCH <= PACK_TILE_WORDS, "tile");
   const int nkt = (a.nKF * (MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32) + TK - 1) / TK;
-  const long long rt = blockIdx.x / nkt;
-  const int kt = blockIdx.x % nkt;
+  const long long rt = bid / nkt;
+  const int kt = (int)(bid % nkt);
   const int tid = threadIdx.x;
   const long long row0 = rt * 32;
   const int k0 = kt * TK;
@@ -264,19 +250,25 @@ __global__ void __launch_bounds__(256) k_pack_f32_tiled(PackArgs a) {
   }
 }
 
+template <int MODE_PK>
+__global__ void __launch_bounds__(256) k_pack_f32_tiled(PackArgs a) {
+  __shared__ float tile[PACK_TILE_WORDS];
+  pack_f32_tile_k<MODE_PK>(a, blockIdx.x, tile);
+}
+
 // Same for sources whose rows are contiguous (unit row stride, e.g. B^T of a
 // row-major B): a 128-row x 32-k tile, read as 512-byte runs along rows and
 // transposed through LDS.
 template <int MODE_PK>
-__global__ void __launch_bounds__(256) k_pack_f32_tiled_rows(PackArgs a) {
+__device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const PackArgs &a, long long bid, float *tile) {
   constexpr int TR = 128, TK = 32, PITCH = TK + 1;
-  __shared__ float tile[TR * PITCH];
+  static_assert(TR * PITCH <= PACK_TILE_WORDS, "tile");
   constexpr int KF = MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32;
   // consecutive blocks take neighbouring row blocks at one k range, so the
   // segments they read complete each other's source rows
   const int nrb = (a.nRT + 3) / 4;
-  const long long rb = blockIdx.x % nrb;   // block of 4 row fragments
-  const int kt = blockIdx.x / nrb;
+  const long long rb = bid % nrb;   // block of 4 row fragments
+  const int kt = (int)(bid / nrb);
   const int tid = threadIdx.x;
   const long long row0 = rb * TR;
   const int k0 = kt * TK;
@@ -340,6 +332,12 @@ __global__ void __launch_bounds__(256) k_pack_f32_tiled_rows(PackArgs a) {
       *(uint4 *)(a.dst + off) = make_uint4(w[0], w[1], w[2], w[3]);
     }
   }
+}
+
+template <int MODE_PK>
+__global__ void __launch_bounds__(256) k_pack_f32_tiled_rows(PackArgs a) {
+  __shared__ float tile[PACK_TILE_WORDS];
+  pack_f32_tile_rows<MODE_PK>(a, blockIdx.x, tile);
 }
 
 // f64 -> f64 (T=16, 2 per lane, KF=8)
@@ -856,26 +854,6 @@ int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc) {
   return 0;
 }
 
-int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
-                      const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, 1LL, 0u);
-  unsigned nb = blocks_for(units);
-  if (nb) {
-    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_F32>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
-    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_SPLIT_BF16>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
-    else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
-  }
-  if (family == NUMC_FAMILY_F32)
-    hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
-  else if (family == NUMC_FAMILY_SPLIT_BF16)
-    hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
-  else
-    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 4, 64, 64, 64, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return numc_hip_fail(e, "batched launch");
-  return 0;
-}
 
 // ===========================================================================
 // configuration tables
@@ -1029,7 +1007,7 @@ __device__ __attribute__((always_inline)) inline void digits_unit(const PackArgs
 #pragma unroll
   for (int p = 0; p < P; p++)
     if (__ballot((w[p][0] | w[p][1] | w[p][2] | w[p][3]) != 0)) mask |= 1 << p;
-  if ((threadIdx.x & 63) == 0 && mask) flags_or(a.flags, mask);
+  if ((threadIdx.x & 63) == 0 && mask && a.flags) flags_or(a.flags, mask);
 }
 
 // one 16-byte chunk of source values (EPC of them) into LDS as raw 32-bit
@@ -1051,12 +1029,12 @@ __device__ __attribute__((always_inline)) inline void load_chunk(const S *src, l
 // integer digit planes, rows with unit k-stride: 32 rows x 128 k tiles
 // staged through LDS so that global reads are 16-byte and coalesced
 template <class S, int P>
-__global__ void __launch_bounds__(256) k_pack_digits_tiled(PackArgs a) {
+__device__ __attribute__((always_inline)) inline void digits_tile_k(const PackArgs &a, long long bid, uint32_t *tile) {
   constexpr int TK = 128, PITCH = TK + 1, EPC = 16 / sizeof(S), CPR = TK / EPC;
-  __shared__ uint32_t tile[32 * PITCH];
+  static_assert(32 * PITCH <= PACK_TILE_WORDS, "tile");
   const int nkt = (a.nKF * 32 + TK - 1) / TK;
-  const long long rt = blockIdx.x / nkt;
-  const int kt = blockIdx.x % nkt;
+  const long long rt = bid / nkt;
+  const int kt = (int)(bid % nkt);
   const int tid = threadIdx.x;
   const long long row0 = rt * 32;
   const int k0 = kt * TK;
@@ -1079,17 +1057,23 @@ __global__ void __launch_bounds__(256) k_pack_digits_tiled(PackArgs a) {
   digits_unit<S, P>(a, t, (rt * a.nKF + kf) * 1024 + lane * 16, kf < a.nKF);
 }
 
+template <class S, int P>
+__global__ void __launch_bounds__(256) k_pack_digits_tiled(PackArgs a) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  digits_tile_k<S, P>(a, blockIdx.x, tile);
+}
+
 // integer digit planes, rows contiguous (unit row stride): 128 rows x 32 k
 template <class S, int P>
-__global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
+__device__ __attribute__((always_inline)) inline void digits_tile_rows(const PackArgs &a, long long bid, uint32_t *tile) {
   constexpr int TR = 128, TK = 32, PITCH = TK + 1, EPC = 16 / sizeof(S), CPK = TR / EPC;
-  __shared__ uint32_t tile[TR * PITCH];
+  static_assert(TR * PITCH <= PACK_TILE_WORDS, "tile");
   // one k fragment (32 k) per tile; consecutive blocks take neighbouring
   // row blocks at one k range, so the segments they read complete each
   // other's source rows
   const int nrb = (a.nRT + 3) / 4;
-  const long long rb = blockIdx.x % nrb;
-  const int kt = blockIdx.x / nrb;
+  const long long rb = bid % nrb;
+  const int kt = (int)(bid / nrb);
   const int tid = threadIdx.x;
   const long long row0 = rb * TR;
   const int k0 = kt * TK;
@@ -1110,6 +1094,63 @@ __global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
 #pragma unroll
   for (int e = 0; e < 16; e++) t[e] = tile[(rl * 32 + (lane & 31)) * PITCH + (lane >> 5) * 16 + e];
   digits_unit<S, P>(a, t, (rt * a.nKF + kt) * 1024 + lane * 16, rt < a.nRT);
+}
+
+template <class S, int P>
+__global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  digits_tile_rows<S, P>(a, blockIdx.x, tile);
+}
+
+// ---------------------------------------------------------------------------
+// batched pack: every operand of every problem in one launch.  Operand i owns
+// blocks [prefix[i], prefix[i+1]); its `mode` (set on the host) says which
+// tile walk it takes: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 one
+// thread per 16-byte lane slot (any other strides).
+// ---------------------------------------------------------------------------
+template <int FAM>
+__global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, const long long *prefix, int ndesc) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  const long long b = blockIdx.x;
+  if (b >= prefix[ndesc]) return;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const PackArgs a = descs[lo];
+  const long long u = b - prefix[lo];
+  constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : 1;
+  if (a.mode == 0) {
+    if constexpr (FAM == NUMC_FAMILY_I8) digits_tile_k<int8_t, 1>(a, u, tile);
+    else pack_f32_tile_k<MODE_PK>(a, u, (float *)tile);
+  } else if (a.mode == 1) {
+    if constexpr (FAM == NUMC_FAMILY_I8) digits_tile_rows<int8_t, 1>(a, u, tile);
+    else pack_f32_tile_rows<MODE_PK>(a, u, (float *)tile);
+  } else {
+    const long long t = u * 256 + threadIdx.x;
+    if (t < (long long)a.nRT * a.nKF * 64) {
+      if constexpr (FAM == NUMC_FAMILY_F32) pack_unit_f32(a, t);
+      else if constexpr (FAM == NUMC_FAMILY_SPLIT_BF16) pack_unit_split(a, t);
+      else pack_unit_i8(a, t);
+    }
+  }
+}
+
+// blocks the batched pack gives one operand, and the walk it takes
+int numc_batch_pack_blocks(int family, PackArgs &pk) {
+  const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
+  if (pk.ks == 1) {
+    pk.mode = 0;
+    return (int)((long long)pk.nRT * cdiv((long long)pk.nKF * kf, 128));
+  }
+  if (pk.rs == 1) {
+    pk.mode = 1;
+    return (int)(cdiv(pk.nRT, 4) * cdiv((long long)pk.nKF * kf, 32));
+  }
+  pk.mode = 2;
+  return (int)cdiv((long long)pk.nRT * pk.nKF * 64, 256);
 }
 
 template <class S, int P> static void launch_digits_p(PackArgs &pk, unsigned nb, hipStream_t st) {
@@ -1328,5 +1369,26 @@ int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p) {
                        (const uint64_t *)p.B, (uint64_t *)p.C, p.M, p.K, p.N, p.rsa, p.csa, p.rsb, p.csb, p.rso, p.cso);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "i64 launch");
+  return 0;
+}
+
+int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
+                      const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, 1LL, 0u);
+  const unsigned nb = (unsigned)units; // blocks (numc_batch_pack_blocks summed)
+  if (nb) {
+    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_F32>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_SPLIT_BF16>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+    else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+  }
+  if (family == NUMC_FAMILY_F32)
+    hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  else if (family == NUMC_FAMILY_SPLIT_BF16)
+    hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  else
+    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 4, 64, 64, 64, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return numc_hip_fail(e, "batched launch");
   return 0;
 }

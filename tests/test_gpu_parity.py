@@ -420,6 +420,50 @@ def test_batched_int8_widen_and_wrap(ctx):
             assert np.array_equal(O.numpy(), want)
 
 
+def test_batched_mixed_int8_uint8_widen(ctx):
+    """int8 and uint8 problems widened to int32 in one batch: uint8 values
+    above 127 must not be read as int8."""
+    rng = np.random.default_rng(19)
+    As, Bs, Os, ref = [], [], [], []
+    for i, (M, K, N) in enumerate([(40, 70, 30), (64, 96, 80), (33, 250, 17)]):
+        dt = np.int8 if i % 2 == 0 else np.uint8
+        info = np.iinfo(dt)
+        a = rng.integers(info.min, info.max, (M, K), dtype=dt, endpoint=True)
+        b = rng.integers(info.min, info.max, (K, N), dtype=dt, endpoint=True)
+        As.append(ctx.from_numpy(a))
+        Bs.append(ctx.from_numpy(b))
+        Os.append(ctx.array((M, N), nc.INT32, zeros=True))
+        ref.append(a.astype(np.int64) @ b.astype(np.int64))
+    nc.check(nc.matmul_batched(ctx, As, Bs, Os, flags=nc.OPT_WIDEN), "batched mixed")
+    for want, O in zip(ref, Os):
+        assert np.array_equal(O.numpy(), want.astype(np.int32))
+
+
+@pytest.mark.parametrize("method", [nc.METHOD_PLAIN, nc.METHOD_SPLIT_BF16])
+def test_batched_strided_operands(ctx, method):
+    """Every pack walk of the batched kernel: transposed A (row-contiguous),
+    transposed B (k-contiguous B^T), column-stepped views (lane slots)."""
+    rng = np.random.default_rng(20)
+    As, Bs, Os, ref = [], [], [], []
+    for M, K, N in [(70, 90, 50), (130, 40, 200), (17, 300, 33)]:
+        at = rng.uniform(-1, 1, (K, M)).astype(np.float32)     # A stored transposed
+        bt = rng.uniform(-1, 1, (N, K)).astype(np.float32)     # B stored transposed
+        bs = rng.uniform(-1, 1, (K, 2 * N)).astype(np.float32)  # B with column step 2
+        A = ctx.from_numpy(at).transpose()
+        B1 = ctx.from_numpy(bt).transpose()
+        B2 = ctx.from_numpy(bs).slice(1, 0, 2 * N, 2)
+        for B, b in ((B1, bt.T), (B2, bs[:, ::2])):
+            As.append(A)
+            Bs.append(B)
+            Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+            ref.append((np.ascontiguousarray(at.T), np.ascontiguousarray(b)))
+    nc.check(nc.matmul_batched(ctx, As, Bs, Os, method), "batched strided")
+    for (a, b), O in zip(ref, Os):
+        K = a.shape[1]
+        bound = (K + 2) * U if method == nc.METHOD_PLAIN else 3 * 2.0 ** -16 + (K + 2) * U
+        assert cw_err(O.numpy(), a, b) <= bound
+
+
 def test_batched_fallback_dtypes_and_empty(ctx):
     rng = np.random.default_rng(18)
     As, Bs, Os, ref = [], [], [], []
