@@ -15,6 +15,9 @@ Errors: max componentwise |C - R| / (|A||B|) over all 67M entries, with R and
 by the test suite), for the N(0,1) inputs and an ill-scaled set (rows of A and
 columns of B scaled by 2^u, u integer uniform in [-20, 20]).
 
+Warm-up: each method first runs for 0.3 s untimed (the clocks ramp up under
+a heavy MFMA load), then the W warmup steps, then the K timed steps.
+
 roofline: the MFMA kernel of the split-BF16 method; its average duration comes
 from HIP events the library records on its own stream around each launch.
 cpu_baseline: numc's own numc_matmul (oracle/_ref, built from numc's sources,
@@ -35,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "GEMM TFLOPS/%peak: split-BF16 vs plain FP32 at ≤2× FP32 err vs FP64 oracle"
 PEAK = {"f32": 157.3, "split_bf16": 2516.0 / 3.0, "fp8": 5030.0, "f64": 78.6}   # TFLOP/s, MI355X dense
 HBM_PEAK = 8000.0  # GB/s
+WARM_S = 0.3       # untimed clock warm-up per method, before the W warmup steps
 
 
 def log(*a):
@@ -48,6 +52,12 @@ def time_method(nc, ctx, A, B, C, method, family, steps, warmup, n):
     cfg = nc.lib().numc_selector_choice(family, n, n, n, 0)
     if cfg < 0:
         nc.check(cfg, "numc_selector_choice")
+    # bring the GPU clocks to their loaded state first: the first ~30 ms of a
+    # heavy MFMA load run up to 15% slower while the clocks ramp
+    t_end = time.perf_counter() + WARM_S
+    while time.perf_counter() < t_end:
+        nc.check(nc.matmul(A, B, C, method), "clock warm-up")
+        ctx.synchronize()
     for _ in range(warmup):
         nc.check(nc.matmul(A, B, C, method), "warmup")
     ctx.synchronize()
