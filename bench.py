@@ -77,6 +77,7 @@ def time_method(nc, ctx, A, B, C, method, family, steps, warmup, n):
     ctx.profile(False)
     return {"config": nc.config_get(family, cfg), "ms_per_step": region / steps, "wall_ms_per_step": wall / steps,
             "kernel_ms": gemm_ms / max(gemm_n, 1), "kernel_launches": gemm_n,
+            "kernel_ms_per_step": gemm_ms / steps,
             "pack_ms": pack_ms / max(pack_n, 1)}
 
 
@@ -164,6 +165,34 @@ def bench_batched(nc, count, seed, reps):
     return res
 
 
+def bench_suite(nc, steps, warmup, seed):
+    """The other reference configurations timed the same way (one JSON line
+    each): float32 512^3, float64 2048^3, int32 2048^3 in [-100,100], int8
+    4096^3 full range; operands resident in HBM, one step = one numc_matmul."""
+    ctx = nc.Ctx()
+    ctx.set_sync(False)
+    peaks = {nc.FLOAT32: ("f32", 157.3, nc.FAMILY_F32), nc.FLOAT64: ("f64", 78.6, nc.FAMILY_F64),
+             nc.INT32: ("i32", 5030.0, nc.FAMILY_I8), nc.INT8: ("i8", 5030.0, nc.FAMILY_I8)}
+    for dt, n, dist, lo, hi in ((nc.FLOAT32, 512, nc.DIST_UNIFORM, -1, 1), (nc.FLOAT64, 2048, nc.DIST_NORMAL, 0, 0),
+                                (nc.INT32, 2048, nc.DIST_INT, -100, 100), (nc.INT8, 4096, nc.DIST_INT, -128, 127)):
+        A, B, Cm = ctx.array((n, n), dt), ctx.array((n, n), dt), ctx.array((n, n), dt, zeros=True)
+        A.fill_random(dist, seed, 0, lo, hi)
+        B.fill_random(dist, seed, 1, lo, hi)
+        for x in (A, B, Cm):
+            x.to_device()
+        name, peak, fam = peaks[dt]
+        t = time_method(nc, ctx, A, B, Cm, nc.METHOD_AUTO, fam, steps, warmup, n)
+        flops = 2.0 * n ** 3
+        print(json.dumps({"suite": name, "n": n, "ms_per_step": round(t["ms_per_step"], 5),
+                          "t_per_s": round(flops / t["ms_per_step"] * 1e-9, 2),
+                          "gemm_kernel_ms_per_step": round(t["kernel_ms_per_step"], 5),
+                          "gemm_launches_per_step": t["kernel_launches"] / steps,
+                          "kernel_t_per_s": round(flops / t["kernel_ms_per_step"] * 1e-9, 2)
+                          if t["kernel_ms_per_step"] else None,
+                          "peak": peak, "config": t["config"]}), flush=True)
+    ctx.close()
+
+
 def read_traffic():
     """HBM bytes per launch of the split-BF16 kernel from the committed rocprof
     summary (FETCH_SIZE/WRITE_SIZE runs scaled by a plain copy of known size)."""
@@ -195,6 +224,8 @@ def main():
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--only", default=None, help="time one method: split_bf16, f32 or fp8")
+    ap.add_argument("--suite", action="store_true",
+                    help="time the other reference configurations (f32 512^3, f64 2048^3, int32 2048^3, int8 4096^3)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the reference copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -216,6 +247,9 @@ def main():
             nc.check(nc.lib().numc_device_copy(ctx.h, dst, src, nbytes), "numc_device_copy")
         print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
         ctx.close()
+        return
+    if args.suite:
+        bench_suite(nc, args.steps, args.warmup, args.seed)
         return
     n = args.n
     flops = 2.0 * n * n * n

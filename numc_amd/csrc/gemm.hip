@@ -432,6 +432,49 @@ __global__ void k_rowmax_colwise(const float *src, long long rs, long long ks, i
   atomicMax(&bits[row], __float_as_uint(m));
 }
 
+// Same maxima with 16-byte loads (the launcher checks alignment): one block
+// per row for k-contiguous rows; 1024 rows x 128 k per block for
+// row-contiguous sources, combined with atomicMax.
+__global__ void __launch_bounds__(256) k_rowmax_rowwise_v4(const float *src, long long rs, int K, uint32_t *bits) {
+  const int row = blockIdx.x;
+  const float4 *p = (const float4 *)(src + row * rs);
+  const int K4 = K >> 2;
+  float m = 0.f;
+  for (int k = threadIdx.x; k < K4; k += 256) {
+    const float4 v = p[k];
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+  }
+  for (int k = (K4 << 2) + threadIdx.x; k < K; k += 256) m = fmaxf(m, fabsf(src[row * rs + k]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) bits[row] = __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
+}
+
+__global__ void __launch_bounds__(256) k_rowmax_colwise_v4(const float *src, long long ks, int R, int K,
+                                                           uint32_t *bits) {
+  constexpr int KS = 128;
+  const int r0 = blockIdx.x * 1024 + threadIdx.x * 4;
+  const int k0 = blockIdx.y * KS, k1 = min(K, k0 + KS);
+  if (r0 >= R) return;
+  float m[4] = {0.f, 0.f, 0.f, 0.f};
+  if (r0 + 3 < R) {
+#pragma unroll 8
+    for (int k = k0; k < k1; k++) {
+      const float4 v = *(const float4 *)(src + (long long)k * ks + r0);
+      m[0] = fmaxf(m[0], fabsf(v.x));
+      m[1] = fmaxf(m[1], fabsf(v.y));
+      m[2] = fmaxf(m[2], fabsf(v.z));
+      m[3] = fmaxf(m[3], fabsf(v.w));
+    }
+  } else {
+    for (int k = k0; k < k1; k++)
+      for (int e = 0; e < 4 && r0 + e < R; e++) m[e] = fmaxf(m[e], fabsf(src[(long long)k * ks + r0 + e]));
+  }
+  for (int e = 0; e < 4 && r0 + e < R; e++) atomicMax(&bits[r0 + e], __float_as_uint(m[e]));
+}
+
 __global__ void k_exps(const uint32_t *bits, int *ex, int n) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) ex[i] = fp8_scale_exp(__uint_as_float(bits[i]));
@@ -1215,6 +1258,16 @@ static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, h
 static void launch_rowmax(const float *src, long long rs, long long ks, int R, int K, uint32_t *bits,
                           hipStream_t st) {
   if (R <= 0) return;
+  const bool al = (((uintptr_t)src) & 15) == 0;
+  if (ks == 1 && al && (rs & 3) == 0) {
+    hipLaunchKernelGGL(k_rowmax_rowwise_v4, dim3(R), dim3(256), 0, st, src, rs, K, bits);
+    return;
+  }
+  if (rs == 1 && al && (ks & 3) == 0) {
+    dim3 grid((unsigned)cdiv(R, 1024), (unsigned)cdiv(K > 0 ? K : 1, 128));
+    hipLaunchKernelGGL(k_rowmax_colwise_v4, grid, dim3(256), 0, st, src, ks, R, K, bits);
+    return;
+  }
   if (ks == 1 || (rs != 1 && K >= 256)) {
     hipLaunchKernelGGL(k_rowmax_rowwise, dim3(R), dim3(256), 0, st, src, rs, ks, R, K, bits);
   } else {
