@@ -19,6 +19,15 @@ extern "C" {
 NUMC_API int numc_matmul(const NumcArray *a, const NumcArray *b,
                          NumcArray *out);
 
+/* numc_dot (math.h:593, src/reduction/dot.c:441-448): numpy-style dot with
+ * numc's validation (src/reduction/dispatch.h:199-283, same codes and
+ * messages).  0-D operand: out = other * scalar; 1-D . 1-D: inner product;
+ * N-D . 1-D and N-D . M-D: sum over a's last axis and b's second-to-last,
+ * out shape a.shape[:-1] + b.shape[:-2] + b.shape[-1:].  Computed by the GPU
+ * GEMM (several problems in one batched launch); integers give the exact sum
+ * mod 2^w. */
+NUMC_API int numc_dot(const NumcArray *a, const NumcArray *b, NumcArray *out);
+
 /* numc_matmul_naive (math.h:603, matmul.c:333): same result contract; it
  * treats a, b and out as contiguous row-major (as numc does) and zeroes out
  * first.  Runs a simple one-thread-per-output GPU kernel. */

@@ -91,6 +91,7 @@ _SIGS = {
     "numc_matmul_naive": (C.c_int, [_P, _P, _P]),
     # numc_amd.h
     "numc_matmul_ex": (C.c_int, [_P, _P, _P, C.POINTER(MatmulOpts)]),
+    "numc_dot": (C.c_int, [_P, _P, _P]),
     "numc_amd_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t,
                                 _P, C.c_size_t, C.c_size_t, _P, C.c_size_t, C.c_size_t,
                                 _P, C.c_size_t, C.c_size_t, C.POINTER(MatmulOpts)]),
@@ -191,7 +192,9 @@ class Ctx:
         return Array(h, self)
 
     def from_numpy(self, x):
-        x = np.ascontiguousarray(x)
+        x = np.asarray(x)
+        if not x.flags.c_contiguous:   # (ascontiguousarray would turn 0-D into 1-D)
+            x = x.copy(order="C")
         a = self.array(x.shape, DTYPE_OF_NP[x.dtype])
         lib().numc_array_write(a.h, x.ctypes.data_as(C.c_void_p))
         return a
@@ -252,8 +255,10 @@ class Array:
         ptr = lib().numc_array_data(self.h)
         shape, strides = self.shape, self.strides
         npdt = np.dtype(NP_DTYPES[self.dtype])
-        if not shape or 0 in shape:
+        if 0 in shape:
             return np.zeros(shape, npdt)
+        if not shape:   # 0-D: one element
+            return np.array(np.frombuffer((C.c_char * npdt.itemsize).from_address(ptr), npdt)[0])
         span = sum((s - 1) * st for s, st in zip(shape, strides)) + npdt.itemsize
         buf = (C.c_char * span).from_address(ptr)
         return np.ndarray(shape, npdt, buffer=buf, strides=strides).copy()
@@ -287,6 +292,11 @@ class Array:
 def matmul(a, b, out, method=METHOD_AUTO, config=-1, flags=0):
     opts = MatmulOpts(method, config, flags)
     return lib().numc_matmul_ex(a.h, b.h, out.h, C.byref(opts))
+
+
+def dot(a, b, out):
+    """numc_dot(a, b, out) -> numc error code"""
+    return lib().numc_dot(a.h, b.h, out.h)
 
 
 def gemm_pointers(ctx, a, b, c, method=METHOD_AUTO, config=-1, flags=0):

@@ -63,6 +63,19 @@ int numc_gemm_config_count(int family);
 size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *aux_bytes);
 int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, void *ws, void *aux);
 int numc_naive_launch(NumcCtx *ctx, NumcDType dt, const void *a, const void *b, void *c, int M, int K, int N);
+
+// out[i] = src[i] * scalar over an N-D strided view (element strides), in the
+// dtype's own arithmetic: integers keep the low w bits of the product
+#define NUMC_EW_MAX_DIMS 16
+struct ScaleArgs {
+  const unsigned char *src;
+  unsigned char *dst;
+  long long n;
+  int ndim, dtype;
+  long long shape[NUMC_EW_MAX_DIMS], ss[NUMC_EW_MAX_DIMS], ds[NUMC_EW_MAX_DIMS];
+  uint64_t scalar_bits; // the scalar in the dtype's representation
+};
+int numc_scale_launch(NumcCtx *ctx, const ScaleArgs &a);
 int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p);
 // heuristic / learned choice of config index for a family and shape
 int numc_select_config(int family, int M, int N, int K, int flags);

@@ -1411,6 +1411,53 @@ int numc_naive_launch(NumcCtx *ctx, NumcDType dt, const void *a, const void *b, 
   return 0;
 }
 
+template <class T> __device__ inline T scale_one(T x, uint64_t bits) {
+  if constexpr (std::is_integral_v<T> && sizeof(T) == 8) {
+    return (T)((uint64_t)x * bits); // low 64 bits
+  } else if constexpr (std::is_integral_v<T>) {
+    return (T)(uint32_t)((uint32_t)(int32_t)x * (uint32_t)bits); // low w bits
+  } else if constexpr (sizeof(T) == 4) {
+    return x * __uint_as_float((uint32_t)bits);
+  } else {
+    return x * __longlong_as_double((long long)bits);
+  }
+}
+
+template <class T>
+__global__ void k_scale(ScaleArgs a) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < a.n; i += (long long)gridDim.x * 256) {
+    long long rem = i, so = 0, dof = 0;
+    for (int d = a.ndim - 1; d >= 0; d--) {
+      const long long q = rem / a.shape[d], r = rem - q * a.shape[d];
+      so += r * a.ss[d];
+      dof += r * a.ds[d];
+      rem = q;
+    }
+    ((T *)a.dst)[dof] = scale_one<T>(((const T *)a.src)[so], a.scalar_bits);
+  }
+}
+
+int numc_scale_launch(NumcCtx *ctx, const ScaleArgs &a) {
+  if (a.n == 0) return 0;
+  const unsigned nb = (unsigned)std::min<long long>(cdiv(a.n, 256), 65536);
+  hipStream_t s = ctx->stream;
+  switch (a.dtype) {
+  case NUMC_DTYPE_INT8: hipLaunchKernelGGL(k_scale<int8_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_INT16: hipLaunchKernelGGL(k_scale<int16_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_INT32: hipLaunchKernelGGL(k_scale<int32_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_INT64: hipLaunchKernelGGL(k_scale<int64_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_UINT8: hipLaunchKernelGGL(k_scale<uint8_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_UINT16: hipLaunchKernelGGL(k_scale<uint16_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_UINT32: hipLaunchKernelGGL(k_scale<uint32_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_UINT64: hipLaunchKernelGGL(k_scale<uint64_t>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_FLOAT32: hipLaunchKernelGGL(k_scale<float>, dim3(nb), dim3(256), 0, s, a); break;
+  default: hipLaunchKernelGGL(k_scale<double>, dim3(nb), dim3(256), 0, s, a); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return numc_hip_fail(e, "scale launch");
+  return 0;
+}
+
 int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p) {
   if (p.M == 0 || p.N == 0) return 0;
   dim3 grid((unsigned)cdiv(p.N, 16), (unsigned)cdiv(p.M, 16));

@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 static int check_matmul(const NumcArray *a, const NumcArray *b, const NumcArray *out, bool widen) {
   if (!a || !b || !out)
@@ -325,4 +326,225 @@ extern "C" NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, si
     return 0;
   }
   return numc_finish(ctx);
+}
+
+// ---- numc_dot ----------------------------------------------------------------------
+//
+// numc's numc_dot (src/reduction/dot.c:441-448; validation
+// src/reduction/dispatch.h:199-283) on the GPU:
+//   0-D with anything    out = other * scalar in the dtype's arithmetic, the
+//                        scalar passing through double as numc's
+//                        _dot_scalar_case -> numc_mul_scalar does
+//                        (dot.c:340-346);
+//   1-D . 1-D            inner product (a 1 x K by K x 1 GEMM);
+//   N-D . 1-D, N-D . M-D sum over a's last axis and b's second-to-last:
+//                        out = a.shape[:-1] + b.shape[:-2] + b.shape[-1:],
+//                        one GEMM per run of a's leading indices that is
+//                        linear in both a and out, and per index of b's
+//                        leading axes; several of them go to the persistent
+//                        batched kernel in one launch.
+// Integers give the exact sum mod 2^w as numc's reduction and naive kernels
+// do (dot.c:22-24, 43-69); floats compute in their own precision.  Where numc
+// flattens a's leading axes with a->strides[0] as the row stride
+// (dot.c:371-372, correct only for 2-D a) and divides by a zero-length last
+// axis (dot.c:368), this follows the numpy semantics its checker describes.
+
+static int check_dot(const NumcArray *a, const NumcArray *b, const NumcArray *out) {
+  if (!a || !b || !out)
+    return NUMC_SET_ERROR(NUMC_ERR_NULL, "dot: NULL pointer (a=%p b=%p out=%p)", (const void *)a, (const void *)b,
+                          (const void *)out);
+  if (a->dtype != b->dtype || a->dtype != out->dtype)
+    return NUMC_SET_ERROR(NUMC_ERR_TYPE, "dot: dtype mismatch (a=%d b=%d out=%d)", a->dtype, b->dtype, out->dtype);
+  if (a->dim == 0 || b->dim == 0) {
+    const NumcArray *other = a->dim == 0 ? b : a;
+    if (out->dim != other->dim)
+      return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "dot: output ndim mismatch (out.dim=%zu other.dim=%zu)", out->dim,
+                            other->dim);
+    for (size_t i = 0; i < out->dim; i++)
+      if (out->shape[i] != other->shape[i])
+        return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "dot: shape mismatch at axis %zu", i);
+    return 0;
+  }
+  const size_t al = a->dim - 1, br = b->dim == 1 ? 0 : b->dim - 2;
+  if (a->shape[al] != b->shape[br])
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "dot: axis mismatch (a.shape[%zu]=%zu b.shape[%zu]=%zu)", al,
+                          a->shape[al], br, b->shape[br]);
+  const size_t expected = (a->dim - 1) + (b->dim > 1 ? b->dim - 1 : 0);
+  const bool scalar_out = expected == 0 && (out->dim == 0 || (out->dim == 1 && out->size == 1));
+  if (!scalar_out && out->dim != expected)
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "dot: output ndim mismatch (out.dim=%zu expected=%zu)", out->dim, expected);
+  if (!scalar_out) {
+    size_t o = 0;
+    bool ok = true;
+    for (size_t i = 0; i + 1 < a->dim; i++) ok = ok && out->shape[o++] == a->shape[i];
+    if (b->dim > 1) {
+      for (size_t i = 0; i + 2 < b->dim; i++) ok = ok && out->shape[o++] == b->shape[i];
+      ok = ok && out->shape[o++] == b->shape[b->dim - 1];
+    }
+    if (!ok) return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "dot: output shape mismatch");
+  }
+  return 0;
+}
+
+// the scalar of a 0-D operand in the dtype's representation after numc's
+// round trip through double (exact except for 64-bit integers beyond 2^53;
+// out-of-range results as the x86-64 conversion sequences give them)
+static int dot_scalar_bits(const NumcArray *sc, uint64_t *bits) {
+  unsigned char raw[8] = {0};
+  if (sc->st->residency == RES_DEVICE) {
+    unsigned char *d = numc_dev_ptr(sc);
+    if (!d) return NUMC_ERR_INTERNAL;
+    HIP_TRY(hipMemcpyAsync(raw, d, sc->elem_size, hipMemcpyDeviceToHost, sc->ctx->stream), "hipMemcpy D2H");
+    HIP_TRY(hipStreamSynchronize(sc->ctx->stream), "hipStreamSynchronize");
+  } else {
+    if (sc->ctx->gpu_ready && !sc->ctx->synchronous)
+      HIP_TRY(hipStreamSynchronize(sc->ctx->stream), "hipStreamSynchronize");
+    memcpy(raw, sc->data, sc->elem_size);
+  }
+  double v = 0;
+  switch (sc->dtype) {
+  case NUMC_DTYPE_INT8: v = *(int8_t *)raw; break;
+  case NUMC_DTYPE_INT16: v = *(int16_t *)raw; break;
+  case NUMC_DTYPE_INT32: v = *(int32_t *)raw; break;
+  case NUMC_DTYPE_INT64: v = (double)*(int64_t *)raw; break;
+  case NUMC_DTYPE_UINT8: v = *(uint8_t *)raw; break;
+  case NUMC_DTYPE_UINT16: v = *(uint16_t *)raw; break;
+  case NUMC_DTYPE_UINT32: v = *(uint32_t *)raw; break;
+  case NUMC_DTYPE_UINT64: v = (double)*(uint64_t *)raw; break;
+  case NUMC_DTYPE_FLOAT32: v = *(float *)raw; break;
+  case NUMC_DTYPE_FLOAT64: v = *(double *)raw; break;
+  }
+  switch (sc->dtype) {
+  case NUMC_DTYPE_INT64:
+    *bits = v >= 9223372036854775808.0 ? (uint64_t)1 << 63 : (uint64_t)(int64_t)v;
+    break;
+  case NUMC_DTYPE_UINT64:
+    *bits = v >= 18446744073709551616.0 ? 0 : (uint64_t)v;
+    break;
+  case NUMC_DTYPE_FLOAT32: {
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    *bits = u;
+    break;
+  }
+  case NUMC_DTYPE_FLOAT64: memcpy(bits, &v, 8); break;
+  case NUMC_DTYPE_UINT8: case NUMC_DTYPE_UINT16: case NUMC_DTYPE_UINT32: *bits = (uint64_t)v; break;
+  default: *bits = (uint64_t)(int64_t)v; break; // int8/16/32: exact
+  }
+  return 0;
+}
+
+static int dot_scalar(const NumcArray *a, const NumcArray *b, NumcArray *out) {
+  const NumcArray *sc = a->dim == 0 ? a : b, *other = a->dim == 0 ? b : a;
+  if (other->size == 0) return 0;
+  if (other->dim > NUMC_EW_MAX_DIMS)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "dot: at most %d dimensions on the GPU path", NUMC_EW_MAX_DIMS);
+  NumcCtx *ctx = out->ctx;
+  int err = numc_gpu_init(ctx);
+  if (err) return err;
+  ScaleArgs s{};
+  if ((err = dot_scalar_bits(sc, &s.scalar_bits))) return err;
+  const void *src;
+  void *dst;
+  if ((err = stage_in(ctx, other, &src)) || (err = stage_out_begin(ctx, out, &dst))) return err;
+  s.src = (const unsigned char *)src;
+  s.dst = (unsigned char *)dst;
+  s.n = (long long)other->size;
+  s.ndim = (int)other->dim;
+  s.dtype = other->dtype;
+  for (size_t i = 0; i < other->dim; i++) {
+    s.shape[i] = (long long)other->shape[i];
+    s.ss[i] = (long long)(other->strides[i] / other->elem_size);
+    s.ds[i] = (long long)(out->strides[i] / out->elem_size);
+  }
+  if ((err = numc_scale_launch(ctx, s))) return err;
+  if ((err = stage_out_end(ctx, out, dst))) return err;
+  return numc_finish(ctx);
+}
+
+// a 2-D view header sharing src's storage (no allocation)
+static void view2(NumcArray *v, const NumcArray *src, size_t byte_off, size_t rows, size_t cols, size_t rstride,
+                  size_t cstride) {
+  memset(v, 0, sizeof *v);
+  v->ctx = src->ctx;
+  v->dtype = src->dtype;
+  v->elem_size = src->elem_size;
+  v->st = src->st;
+  v->dim = 2;
+  v->shape = v->shape_buf;
+  v->strides = v->strides_buf;
+  v->shape[0] = rows;
+  v->shape[1] = cols;
+  v->strides[0] = rstride;
+  v->strides[1] = cstride;
+  v->size = rows * cols;
+  v->capacity = v->size;
+  v->is_contiguous = cstride == v->elem_size && (rows <= 1 || rstride == cols * v->elem_size);
+  v->data = (unsigned char *)src->data + byte_off;
+}
+
+static int dot_gemm(const NumcArray *a, const NumcArray *b, NumcArray *out) {
+  const size_t es = a->elem_size;
+  const size_t k = a->shape[a->dim - 1];
+  const size_t n = b->dim == 1 ? 1 : b->shape[b->dim - 1];
+  const size_t na = a->dim - 1;                    // a's leading axes (out axes 0..na-1)
+  const size_t nbb = b->dim >= 2 ? b->dim - 2 : 0; // b's batch axes (out axes na..na+nbb-1)
+  const size_t csa = a->strides[a->dim - 1];
+  const size_t rsb = b->strides[b->dim == 1 ? 0 : b->dim - 2];
+  const size_t csb = b->dim == 1 ? es : b->strides[b->dim - 1];
+  const size_t cso = b->dim == 1 ? es : out->strides[out->dim - 1];
+  if (out->size == 0) return 0;
+
+  // the trailing run of a's leading axes that is linear in a and in out
+  size_t c = na;
+  if (na > 0) {
+    c = na - 1;
+    while (c > 0 && a->strides[c - 1] == a->strides[c] * a->shape[c] &&
+           out->strides[c - 1] == out->strides[c] * out->shape[c])
+      c--;
+  }
+  size_t M = 1;
+  for (size_t i = c; i < na; i++) M *= a->shape[i];
+  const size_t rsa = na > 0 ? a->strides[na - 1] : k * es;
+  const size_t rso = na > 0 ? out->strides[na - 1] : n * es;
+
+  // every combination of the outer a axes and of b's batch axes is a problem
+  size_t n_outer = 1, n_batch = 1;
+  for (size_t i = 0; i < c; i++) n_outer *= a->shape[i];
+  for (size_t i = 0; i < nbb; i++) n_batch *= b->shape[i];
+  const size_t count = n_outer * n_batch;
+  std::vector<NumcArray> views(3 * count);
+  std::vector<NumcArray *> pa(count), pb(count), po(count);
+  for (size_t t = 0; t < count; t++) {
+    size_t oi = t / n_batch, bi = t % n_batch;
+    size_t aoff = 0, boff = 0, ooff = 0;
+    for (size_t i = c; i-- > 0;) { // outer a axes
+      const size_t idx = oi % a->shape[i];
+      oi /= a->shape[i];
+      aoff += idx * a->strides[i];
+      ooff += idx * out->strides[i];
+    }
+    for (size_t i = nbb; i-- > 0;) { // b's batch axes
+      const size_t idx = bi % b->shape[i];
+      bi /= b->shape[i];
+      boff += idx * b->strides[i];
+      ooff += idx * out->strides[na + i];
+    }
+    view2(&views[3 * t], a, aoff, M, k, rsa, csa);
+    view2(&views[3 * t + 1], b, boff, k, n, rsb, csb);
+    view2(&views[3 * t + 2], out, ooff, M, n, rso, cso);
+    pa[t] = &views[3 * t];
+    pb[t] = &views[3 * t + 1];
+    po[t] = &views[3 * t + 2];
+  }
+  if (count == 1) return numc_matmul_one(pa[0], pb[0], po[0], nullptr, true);
+  return numc_matmul_batched(out->ctx, count, pa.data(), pb.data(), po.data(), nullptr);
+}
+
+extern "C" NUMC_API int numc_dot(const NumcArray *a, const NumcArray *b, NumcArray *out) {
+  int err = check_dot(a, b, out);
+  if (err) return err;
+  if (a->dim == 0 || b->dim == 0) return dot_scalar(a, b, out);
+  return dot_gemm(a, b, out);
 }

@@ -8,6 +8,8 @@ Sources (numc tree):
   tests/matmul/test_matmul.c:170-586        packed-vs-naive cross checks
   tests/matmul/test_matmul.c:743-801        edge-shape sweep and tolerances
   tests/reduction/test_dot_nd.c:47-72       2-D dot closed form
+  tests/reduction/test_dot_nd.c:11-158      numc_dot cases (0-D, 1-D, 2-D, N-D)
+  tests/reduction/test_dot.c:10-28          1-D inner product
 Run: python tests/golden/make_known_answers.py
 """
 import json
@@ -16,6 +18,23 @@ import os
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "numc_known_answers.json")
 
 data = {
+    # numc_dot: shapes (a, b, out), data as flat lists ("fill" v: every
+    # element v), expected flat output, float32
+    "dot": [
+        {"case": "0-D . 1-D", "a_shape": [], "a": [2.0], "b_shape": [4], "b": [1, 2, 3, 4],
+         "out_shape": [4], "expect": [2, 4, 6, 8], "source": "test_dot_nd.c:11-28"},
+        {"case": "1-D . 1-D", "a_shape": [3], "a": [1, 2, 3], "b_shape": [3], "b": [4, 5, 6],
+         "out_shape": [1], "expect": [32], "source": "test_dot_nd.c:30-46"},
+        {"case": "1-D . 1-D", "a_shape": [4], "a": [1, 2, 3, 4], "b_shape": [4], "b": [2, 3, 4, 5],
+         "out_shape": [1], "expect": [40], "source": "test_dot.c:10-28"},
+        {"case": "2-D . 2-D", "a_shape": [2, 3], "a": [1, 2, 3, 4, 5, 6], "b_shape": [3, 2],
+         "b": [7, 8, 9, 10, 11, 12], "out_shape": [2, 2], "expect": [58, 64, 139, 154],
+         "source": "test_dot_nd.c:48-72"},
+        {"case": "2-D . 1-D", "a_shape": [2, 3], "a": [1, 2, 3, 4, 5, 6], "b_shape": [3], "b": [1, 1, 1],
+         "out_shape": [2], "expect": [6, 15], "source": "test_dot_nd.c:74-94"},
+        {"case": "2-D . 3-D", "a_shape": [2, 3], "a": "fill 1", "b_shape": [4, 3, 5], "b": "fill 1",
+         "out_shape": [2, 4, 5], "expect": "fill 3", "source": "test_dot_nd.c:96-120"},
+    ],
     "closed_form": [
         # a (2x3), b (3x2), expected (2x2), dtypes
         {"a": [[1, 2, 3], [4, 5, 6]], "b": [[7, 8], [9, 10], [11, 12]],
