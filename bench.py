@@ -139,6 +139,10 @@ def bench_batched(nc, count, seed, reps):
             As.append(A)
             Bs.append(B)
             Os.append(O)
+        # autotuned schedule: the 128x128-tile threshold measured on this batch
+        thresh = nc.matmul_batched_tune(ctx, As, Bs, Os, method, flags, 3)
+        if thresh < 0:
+            nc.check(thresh, "numc_matmul_batched_tune")
         nc.check(nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), "batched warmup")  # builds the graph
         ctx.synchronize()
         ctx.mark(2)
@@ -156,6 +160,7 @@ def bench_batched(nc, count, seed, reps):
         ctx.mark(5)
         t_sep = ctx.elapsed_ms(4, 5)
         res[label] = {"ms_batched": round(t_batched, 4), "tflops_batched": round(res["flop"] / t_batched * 1e-9, 2),
+                      "tile_threshold": thresh,
                       "ms_separate_launches": round(t_sep, 3),
                       "tflops_separate": round(res["flop"] / t_sep * 1e-9, 2),
                       "speedup": round(t_sep / t_batched, 2)}

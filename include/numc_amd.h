@@ -59,6 +59,14 @@ NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, size_t M, siz
                            const void *a, size_t rsa, size_t csa, const void *b, size_t rsb, size_t csb,
                            void *c, size_t rso, size_t cso, const NumcMatmulOpts *opts);
 
+/* Autotune the batched schedule for this batch: times the whole batch with
+ * each candidate threshold for 128x128 tiles (a problem takes them when M and
+ * N both exceed it, else 64x64) and keeps the fastest for the batch's dtype
+ * family in later numc_matmul_batched calls.  Returns the threshold. */
+NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, NumcArray *const *a,
+                                      NumcArray *const *b, NumcArray *const *out,
+                                      const NumcMatmulOpts *opts, int reps);
+
 /* number of times the context replayed its cached batched graph */
 NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx);
 

@@ -97,6 +97,8 @@ _SIGS = {
                                 _P, C.c_size_t, C.c_size_t, C.POINTER(MatmulOpts)]),
     "numc_matmul_batched": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                       C.POINTER(MatmulOpts)]),
+    "numc_matmul_batched_tune": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                                           C.POINTER(MatmulOpts), C.c_int]),
     "numc_ctx_batch_graph_replays": (C.c_longlong, [_P]),
     "numc_autotune_release": (C.c_int, [_P]),
     "numc_device_copy": (C.c_int, [_P, _P, _P, _SZ]),
@@ -310,6 +312,15 @@ def gemm_pointers(ctx, a, b, c, method=METHOD_AUTO, config=-1, flags=0):
                                a.ctypes.data, a.strides[0], a.strides[1],
                                b.ctypes.data, b.strides[0], b.strides[1],
                                c.ctypes.data, c.strides[0], c.strides[1], C.byref(opts))
+
+
+def matmul_batched_tune(ctx, As, Bs, Outs, method=METHOD_AUTO, flags=0, reps=3):
+    n = len(As)
+    pa = (C.c_void_p * n)(*[x.h for x in As])
+    pb = (C.c_void_p * n)(*[x.h for x in Bs])
+    po = (C.c_void_p * n)(*[x.h for x in Outs])
+    opts = MatmulOpts(method, -1, flags)
+    return lib().numc_matmul_batched_tune(ctx.h, n, pa, pb, po, C.byref(opts), reps)
 
 
 def matmul_batched(ctx, As, Bs, Outs, method=METHOD_AUTO, config=-1, flags=0):

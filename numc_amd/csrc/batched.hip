@@ -7,7 +7,9 @@
 //   * a pack descriptor per operand (all packed by one launch),
 //   * a problem table (packed operand addresses, output view),
 //   * a work list of (problem, tile row, tile col, tile class) items ordered
-//     longest-first (tile class chosen per problem, see numc_batch_class),
+//     longest-first; a problem takes 128x128 tiles when M and N both exceed
+//     the family's threshold, else 64x64 (numc_batch_choice, selector.cpp),
+//     and numc_matmul_batched_tune picks that threshold by timing the batch,
 // and captures fill-counter -> batched pack -> persistent GEMM into a graph.
 // The graph, its tables and its workspace are cached on the context and
 // replayed while the same arrays (same addresses, shapes, strides) come back.
@@ -21,7 +23,6 @@
 
 int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts,
                     bool finish); // matmul.cpp
-int numc_batch_class(int family, int M, int N, int K); // selector.cpp
 
 namespace {
 struct BatchCache {
@@ -108,12 +109,14 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   const size_t ein = a[0]->elem_size, eout = out[0]->elem_size;
   struct Prob {
     int M, N, K, cls, bm, bn, kc;
+    float pred; // predicted time (orders the work queue)
     const void *A, *B;
     void *C;
     long long rsa, csa, rsb, csb, rso, cso;
     long long pla, plb, offA, offB;
     int nRTa, nRTb, nKF;
   };
+  const int thresh = numc_batch_threshold(family);
   std::vector<Prob> P(count);
   long long packed = 0;
   for (size_t i = 0; i < count; i++) {
@@ -130,6 +133,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     p.A = p.B = nullptr;
     p.C = nullptr;
     p.cls = p.bm = p.bn = p.kc = 0;
+    p.pred = 0.f;
     if (p.M == 0 || p.N == 0) continue;
     if ((err = stage_in(ctx, a[i], &p.A)) || (err = stage_in(ctx, b[i], &p.B))) return err;
     unsigned char *dc = numc_dev_ptr(out[i]);
@@ -137,7 +141,9 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     if (out[i]->st->residency == RES_HOST && numc_view_span(out[i]) > out[i]->size * eout)
       HIP_TRY(hipMemcpyAsync(dc, out[i]->data, numc_view_span(out[i]), hipMemcpyHostToDevice, ctx->stream), "H2D");
     p.C = dc;
-    p.cls = numc_batch_class(family, p.M, p.N, p.K);
+    const BatchChoice ch = numc_batch_choice(family, p.M, p.N, p.K, thresh);
+    p.cls = ch.cls;
+    p.pred = ch.ms;
     numc_batch_class_shape(family, p.cls, &p.bm, &p.bn, &p.kc);
     const long long kp = cdivll(p.K > 0 ? p.K : 1, p.kc) * p.kc;
     p.nKF = (int)(kp / kf);
@@ -154,9 +160,13 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   // ---- tables: work items longest-first ----
   std::vector<size_t> order(count);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-    return (long long)P[x].K * P[x].bm * P[x].bn > (long long)P[y].K * P[y].bm * P[y].bn;
-  });
+  // longest predicted tile first (a problem's time spread over its tiles)
+  auto tile_time = [&](size_t i) {
+    const Prob &p = P[i];
+    const double tiles = (double)cdivll(p.M, p.bm ? p.bm : 1) * cdivll(p.N, p.bn ? p.bn : 1);
+    return p.pred / (tiles > 0 ? tiles : 1.0);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tile_time(x) > tile_time(y); });
   std::vector<int4> items;
   std::vector<PackArgs> descs;
   std::vector<long long> prefix(1, 0);
@@ -190,6 +200,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   std::vector<uint64_t> key;
   key.reserve(4 + count * 12);
   key.push_back((uint64_t)family);
+  key.push_back((uint64_t)(uint32_t)thresh);
   key.push_back((uint64_t)out_dt);
   key.push_back(count);
   for (size_t i = 0; i < count; i++) {
@@ -285,4 +296,44 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     if (span) HIP_TRY(hipMemcpyAsync(out[i]->data, p.C, span, hipMemcpyDeviceToHost, ctx->stream), "D2H");
   }
   return numc_finish(ctx);
+}
+
+// Autotune the batch schedule: time the whole batch (graph replays) with the
+// 128x128/64x64 threshold at each candidate and keep the fastest for the
+// family.  Returns the chosen threshold.
+extern "C" NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, NumcArray *const *a, NumcArray *const *b,
+                                                 NumcArray *const *out, const NumcMatmulOpts *opts, int reps) {
+  if (!ctx || !count || !a || !b || !out) return NUMC_SET_ERROR(NUMC_ERR_NULL, "matmul_batched_tune: NULL argument");
+  const int family = numc_matmul_family(a[0], b[0], out[0], opts);
+  if (family < 0) return family;
+  if (family >= NUMC_FAMILY_COUNT) return NUMC_SET_ERROR(NUMC_ERR_TYPE, "matmul_batched_tune: no batched kernel for this dtype");
+  int err = numc_gpu_init(ctx);
+  if (err) return err;
+  static const int cand[] = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30};
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+  int best = numc_batch_threshold(family);
+  float best_ms = 0.f;
+  for (int t : cand) {
+    numc_batch_set_threshold(family, t);
+    if ((err = numc_matmul_batched(ctx, count, a, b, out, opts))) break; // builds this schedule's graph
+    if ((err = (int)hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : NUMC_ERR_INTERNAL)) break;
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < std::max(1, reps); r++)
+      if ((err = numc_matmul_batched(ctx, count, a, b, out, opts))) break;
+    if (err) break;
+    (void)hipEventRecord(e1, ctx->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (best_ms == 0.f || ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  numc_batch_set_threshold(family, best);
+  return err ? err : best;
 }

@@ -52,6 +52,13 @@ struct BatchProblem {
 };
 
 int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc);
+struct BatchChoice {
+  int cls;  // 0 = 128x128 tiles, 1 = 64x64
+  float ms; // predicted time of the whole problem
+};
+BatchChoice numc_batch_choice(int family, int M, int N, int K, int thresh);
+int numc_batch_threshold(int family);
+void numc_batch_set_threshold(int family, int t);
 int numc_batch_pack_blocks(int family, PackArgs &pk); // sets pk.mode
 int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
                       const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,

@@ -289,9 +289,30 @@ int numc_selector_choose(int family, int M, int N, int K, int flags) {
   return best;
 }
 
-// tile class of a problem in the persistent batched kernel: 0 = 128x128, 1 = 64x64
-int numc_batch_class(int family, int M, int N, int K) {
+// Tile class of one problem of the persistent batched kernel and its
+// estimated time (padded multiply-adds; it orders the work queue
+// longest-first).  A problem takes 128x128 tiles when both M and N exceed the
+// family's threshold, else 64x64.  The threshold is 64 until
+// numc_matmul_batched_tune measures a better one on a real batch: the cost
+// model, trained on single calls, favours small tiles that keep one problem's
+// GPU busy, which is not what a full batch needs.
+static int g_batch_thresh[NUMC_FAMILY_COUNT] = {64, 64, 64, 64, 64};
+
+int numc_batch_threshold(int family) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return family >= 0 && family < NUMC_FAMILY_COUNT ? g_batch_thresh[family] : 64;
+}
+
+void numc_batch_set_threshold(int family, int t) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (family >= 0 && family < NUMC_FAMILY_COUNT) g_batch_thresh[family] = t;
+}
+
+BatchChoice numc_batch_choice(int family, int M, int N, int K, int thresh) {
   (void)family;
-  (void)K;
-  return (M > 64 && N > 64) ? 0 : 1;
+  BatchChoice r;
+  r.cls = (M > thresh && N > thresh) ? 0 : 1;
+  const int t = r.cls == 0 ? 128 : 64;
+  r.ms = (float)(std::ceil(M / (double)t) * std::ceil(N / (double)t) * t * t * (double)(K > 0 ? K : 1) * 1e-9);
+  return r;
 }

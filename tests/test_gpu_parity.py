@@ -464,6 +464,27 @@ def test_batched_strided_operands(ctx, method):
         assert cw_err(O.numpy(), a, b) <= bound
 
 
+def test_batched_tune_keeps_results(ctx):
+    """the schedule autotuner returns one of its thresholds and the batch
+    gives the same (bit-exact) integer results under the tuned schedule"""
+    rng = np.random.default_rng(22)
+    As, Bs, Os, ref = [], [], [], []
+    for M, K, N in [(40, 300, 70), (128, 64, 200), (3, 5, 7), (260, 100, 90), (90, 77, 65)]:
+        a = rng.integers(-128, 127, (M, K), dtype=np.int8, endpoint=True)
+        b = rng.integers(-128, 127, (K, N), dtype=np.int8, endpoint=True)
+        As.append(ctx.from_numpy(a))
+        Bs.append(ctx.from_numpy(b))
+        Os.append(ctx.array((M, N), nc.INT32, zeros=True))
+        ref.append(a.astype(np.int64) @ b.astype(np.int64))
+    t = nc.matmul_batched_tune(ctx, As, Bs, Os, flags=nc.OPT_WIDEN, reps=2)
+    assert t in (0, 32, 48, 64, 96, 128, 192, 256, 1 << 30)
+    for O, want in zip(Os, ref):
+        O.write(np.full(want.shape, -1, np.int32))   # results must be recomputed
+    nc.check(nc.matmul_batched(ctx, As, Bs, Os, flags=nc.OPT_WIDEN), "batched after tune")
+    for want, O in zip(ref, Os):
+        assert np.array_equal(O.numpy(), want.astype(np.int32))
+
+
 def test_batched_fallback_dtypes_and_empty(ctx):
     rng = np.random.default_rng(18)
     As, Bs, Os, ref = [], [], [], []
