@@ -131,6 +131,25 @@ def test_f32_configs(ctx, cfg):
         assert cw_err(got, a, b) <= (K + 2) * U, (cfg, M, K, N)
 
 
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F32)))
+def test_f32_configs_strided(ctx, cfg):
+    """every FP32 configuration (packed and direct) on transposed / stepped
+    operand views and an output with gaps, ragged shapes"""
+    rng = np.random.default_rng(150 + cfg)
+    for M, K, N in [(67, 45, 93), (130, 257, 33), (1, 19, 300)]:
+        at = rng.standard_normal((K, M)).astype(np.float32)       # A stored transposed
+        bs = rng.standard_normal((K, 2 * N + 1)).astype(np.float32)
+        A = ctx.from_numpy(at).transpose()
+        B = ctx.from_numpy(bs).slice(1, 1, 2 * N + 1, 2)            # every other column
+        O_full = ctx.from_numpy(np.full((M, N + 3), 7, np.float32))
+        O = O_full.slice(1, 1, N + 1)
+        nc.check(nc.matmul(A, B, O, nc.METHOD_PLAIN, cfg), "strided f32")
+        full = O_full.numpy()
+        a, b = np.ascontiguousarray(at.T), np.ascontiguousarray(bs[:, 1::2])
+        assert cw_err(np.ascontiguousarray(full[:, 1:N + 1]), a, b) <= (K + 2) * U, (cfg, M, K, N)
+        assert np.all(full[:, 0] == 7) and np.all(full[:, N + 1:] == 7)
+
+
 @pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_SPLIT_BF16)))
 def test_split_bf16_configs(ctx, cfg):
     rng = np.random.default_rng(200 + cfg)
