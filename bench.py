@@ -10,8 +10,9 @@ TFLOP/s; extra MFMA products are not counted as work).
 Configuration: the one numc_matmul runs for the shape (the committed
 exact-shape table numc_amd/data/tuned.csv, from exhaustive autotune).
 
-Errors: max componentwise |C - R| / (|A||B|) over all 67M entries, with R and
-|A||B| from the library's own FP64 MFMA GEMM (validated against the CPU oracle
+Errors: max componentwise |C - R| / (|A||B|) over all 67M entries (also the
+normwise max|C - R| / max|R| and the plain relative |C - R| / |R| over the
+entries with |R| > 1e-3 max|R|), with R and |A||B| from the library's own FP64 MFMA GEMM (validated against the CPU oracle
 by the test suite), for the N(0,1) inputs and an ill-scaled set (rows of A and
 columns of B scaled by 2^u, u integer uniform in [-20, 20]).
 
@@ -91,11 +92,16 @@ def errors_vs_fp64(nc, ctx, a, b, outs, n):
         X, Y = ctx.from_numpy(x), ctx.from_numpy(y)
         nc.check(nc.matmul(X, Y, dst), "fp64 reference")
     r, s = R.numpy(), S.numpy()
+    ar = np.abs(r)
+    tau = 1e-3 * float(np.max(ar))
+    big = (ar > tau) & (ar > 0)
     for name, c in outs.items():
         d = np.abs(c.astype(np.float64) - r)
         with np.errstate(divide="ignore", invalid="ignore"):
             q = np.where(s > 0, d / s, np.where(d > 0, np.inf, 0.0))
-        res[name] = {"componentwise": float(np.max(q)), "normwise": float(np.max(d) / np.max(np.abs(r)))}
+        res[name] = {"componentwise": float(np.max(q)), "normwise": float(np.max(d) / np.max(ar)),
+                     # plain relative error |C - R| / |R| over the entries with |R| > 1e-3 max|R|
+                     "elementwise_rel": float(np.max(d[big] / ar[big]))}
     return res
 
 
@@ -317,8 +323,9 @@ def main():
         log("bench: errors against the FP64 reference (ill-scaled inputs)")
         errs["ill_scaled"] = errors_vs_fp64(nc, ctx, ai, bi, outs_i, n)
         for name in variants:
-            variants[name]["err_componentwise"] = errs["normal"][name]["componentwise"]
-            variants[name]["err_componentwise_ill_scaled"] = errs["ill_scaled"][name]["componentwise"]
+            for kind in ("componentwise", "normwise", "elementwise_rel"):
+                variants[name]["err_" + kind] = errs["normal"][name][kind]
+                variants[name]["err_" + kind + "_ill_scaled"] = errs["ill_scaled"][name][kind]
 
     cpu = None
     if not args.no_cpu:

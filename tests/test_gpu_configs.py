@@ -112,3 +112,4 @@ def test_bench_error_metric_matches_oracle(ctx):
         assert want[0] > 0
         assert got[name]["componentwise"] == pytest.approx(want[0], rel=1e-6), name
         assert got[name]["normwise"] == pytest.approx(want[1], rel=1e-6), name
+        assert got[name]["elementwise_rel"] == pytest.approx(want[2], rel=1e-6), name
