@@ -37,13 +37,35 @@ typedef struct {
 NUMC_API int numc_matmul_ex(const NumcArray *a, const NumcArray *b, NumcArray *out,
                             const NumcMatmulOpts *opts);
 
-/* Many independent problems out[i] = a[i] @ b[i] (no counterpart in numc; it
- * is the batched client ROADMAP.md names as numc_bmm).  All problems share
+/* Many independent problems out[i] = a[i] @ b[i] (no counterpart in numc;
+ * numc_bmm and numc_dot's batched case run on it).  All problems share
  * one dtype (and method); shapes may differ.  Runs as one persistent kernel
  * launch, replayed from a cached HIP graph when the same arrays repeat. */
 NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArray *const *a,
                                  NumcArray *const *b, NumcArray *const *out,
                                  const NumcMatmulOpts *opts);
+
+/* Fused linear layer out = x @ w + bias, the numc_linear(x, w, bias, out) of
+ * numc's roadmap (ROADMAP.md:139; numc has no implementation).  x is
+ * (..., K), w (K, N), bias (N) or NULL, out (..., N), one dtype throughout.
+ * The bias is added in the GEMM's store, on the accumulator before it is
+ * written: floats round once more (fl(acc + bias)), integers keep the exact
+ * sum mod 2^w.  Leading axes of x that are linear in x and out fold into the
+ * GEMM's M; others run as separate GEMMs.  Errors as numc_matmul. */
+NUMC_API int numc_linear(const NumcArray *x, const NumcArray *w, const NumcArray *bias, NumcArray *out);
+
+/* numc_linear with a kernel configuration (opts->config, -1 = selector) and
+ * method AUTO or PLAIN (others: NUMC_ERR_VALUE; no counterpart in numc). */
+NUMC_API int numc_linear_ex(const NumcArray *x, const NumcArray *w, const NumcArray *bias, NumcArray *out,
+                            const NumcMatmulOpts *opts);
+
+/* Batched matrix multiply out[...] = a[...] @ b[...], the numc_bmm(a, b, out)
+ * of numc's roadmap (ROADMAP.md:140; numc has no implementation).  a is
+ * (..., M, K), b (..., K, N), both at least 2-D; the batch axes broadcast as
+ * numpy.matmul's do (aligned from the right, equal or 1) and out is
+ * (batch..., M, N).  All slices run in one persistent numc_matmul_batched
+ * launch (a single slice runs as numc_matmul). */
+NUMC_API int numc_bmm(const NumcArray *a, const NumcArray *b, NumcArray *out);
 
 /* Pointer-level GEMM for a host library that keeps its own arrays: the call
  * numc's numc_matmul (src/matmul/matmul.c:347-389) makes to its dtype kernel

@@ -92,6 +92,9 @@ _SIGS = {
     # numc_amd.h
     "numc_matmul_ex": (C.c_int, [_P, _P, _P, C.POINTER(MatmulOpts)]),
     "numc_dot": (C.c_int, [_P, _P, _P]),
+    "numc_linear": (C.c_int, [_P, _P, _P, _P]),
+    "numc_linear_ex": (C.c_int, [_P, _P, _P, _P, C.POINTER(MatmulOpts)]),
+    "numc_bmm": (C.c_int, [_P, _P, _P]),
     "numc_amd_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t,
                                 _P, C.c_size_t, C.c_size_t, _P, C.c_size_t, C.c_size_t,
                                 _P, C.c_size_t, C.c_size_t, C.POINTER(MatmulOpts)]),
@@ -278,9 +281,9 @@ class Array:
     def to_host(self):
         check(lib().numc_array_to_host(self.h), "numc_array_to_host")
 
-    def transpose(self):
-        axes = _sizes([1, 0])
-        check(lib().numc_array_transpose(self.h, axes), "numc_array_transpose")
+    def transpose(self, axes=(1, 0)):
+        """numc_array_transpose in place (axes: the new order, one entry per axis)"""
+        check(lib().numc_array_transpose(self.h, _sizes(list(axes))), "numc_array_transpose")
         return self
 
     def slice(self, axis, start, stop, step=1):
@@ -299,6 +302,21 @@ def matmul(a, b, out, method=METHOD_AUTO, config=-1, flags=0):
 def dot(a, b, out):
     """numc_dot(a, b, out) -> numc error code"""
     return lib().numc_dot(a.h, b.h, out.h)
+
+
+def linear(x, w, bias, out, method=METHOD_AUTO, config=-1):
+    """numc_linear(x, w, bias, out): out = x @ w + bias (bias may be None) -> error code;
+    a method or configuration goes through numc_linear_ex"""
+    bh = bias.h if bias is not None else None
+    if method == METHOD_AUTO and config < 0:
+        return lib().numc_linear(x.h, w.h, bh, out.h)
+    opts = MatmulOpts(method, config, 0)
+    return lib().numc_linear_ex(x.h, w.h, bh, out.h, C.byref(opts))
+
+
+def bmm(a, b, out):
+    """numc_bmm(a, b, out): batched a @ b with broadcast batch axes -> error code"""
+    return lib().numc_bmm(a.h, b.h, out.h)
 
 
 def gemm_pointers(ctx, a, b, c, method=METHOD_AUTO, config=-1, flags=0):

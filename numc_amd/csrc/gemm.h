@@ -18,6 +18,8 @@ struct GemmProblem {
   long long rsb, csb;
   void *C;
   long long rso, cso;
+  const void *bias;      // optional: out[i][j] += bias[j * bias_s] (output dtype), fused into the store
+  long long bias_s;
 };
 
 struct GemmArgs;
@@ -27,6 +29,7 @@ struct GemmConfig {
   NumcKernelConfig cfg;
   int T; // rows per MFMA fragment
   GemmLaunchFn launch;
+  GemmLaunchFn launch_bias; // same kernel with a per-column bias in the store (float families), or null
 };
 
 struct PackArgs {
@@ -38,7 +41,8 @@ struct PackArgs {
   long long plane_bytes;
   const int *row_exp; // FP8
   int *flags;         // integer digit planes: nonzero mask
-  int mode;           // batched pack: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 lane slots
+  int mode;           // batched pack: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 lane slots,
+                      // 3 int8 byte tiles (k-contiguous), 4 int8 byte tiles (row-contiguous)
 };
 
 // one problem of the persistent batched kernel (packed operands in a workspace)

@@ -65,13 +65,16 @@ __global__ void k_fill_u32(uint32_t *p, long long n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
-// out[i*rso + j*cso] = 0 for an M x N strided view (element size es)
-__global__ void k_zero_strided(unsigned char *c, int M, int N, long long rso, long long cso, int es) {
+// out[i*rso + j*cso] = bias[j*bias_s], or 0 without a bias, for an M x N
+// strided view (element size es): the product of an empty sum
+__global__ void k_zero_strided(unsigned char *c, int M, int N, long long rso, long long cso, int es,
+                               const unsigned char *bias, long long bias_s) {
   long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t >= (long long)M * N) return;
   long long i = t / N, j = t % N;
   unsigned char *p = c + (i * rso + j * cso) * es;
-  for (int b = 0; b < es; b++) p[b] = 0;
+  const unsigned char *q = bias ? bias + j * bias_s * es : nullptr;
+  for (int b = 0; b < es; b++) p[b] = q ? q[b] : 0;
 }
 
 // ===========================================================================
@@ -176,7 +179,7 @@ __global__ void k_pack_fp8(PackArgs a) {
 // row segments (coalesced), writes are whole 1 KiB fragments.  MODE_PK: 0
 // plain f32 (KF 8), 1 bf16 hi/lo (KF 16), 2 e4m3 with row exponents (KF 32).
 // LDS words one pack tile needs (every tiled pack below fits)
-#define PACK_TILE_WORDS (128 * 33 + 64)
+#define PACK_TILE_WORDS (128 * 36)
 
 // tile `bid` of a k-contiguous operand: 32 rows x 128 k through LDS
 template <int MODE_PK>
@@ -190,26 +193,41 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
   const long long row0 = rt * 32;
   const int k0 = kt * TK;
   // 32 rows x 128 floats = 1024 float4, 4 per thread
+  float4 v[4];
+  const float *base = (const float *)a.src + row0 * a.rs + k0;
+  if (row0 + 32 <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.rs * 4)) & 15) == 0) {
+    // interior tile, 16-byte aligned rows: every load in flight at once
 #pragma unroll
-  for (int it = 0; it < 4; it++) {
-    int idx = tid + it * 256;
-    int r = idx >> 5, c4 = (idx & 31) * 4;
-    long long row = row0 + r;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (row < a.R) {
-      const float *src = (const float *)a.src + row * a.rs;
-      int k = k0 + c4;
-      if (k + 3 < a.K && ((((uintptr_t)(src + k)) & 15) == 0)) {
-        v = *(const float4 *)(src + k);
-      } else {
-        if (k < a.K) v.x = src[k];
-        if (k + 1 < a.K) v.y = src[k + 1];
-        if (k + 2 < a.K) v.z = src[k + 2];
-        if (k + 3 < a.K) v.w = src[k + 3];
+    for (int it = 0; it < 4; it++) {
+      const int idx = tid + it * 256;
+      v[it] = *(const float4 *)(base + (idx >> 5) * a.rs + (idx & 31) * 4);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      int idx = tid + it * 256;
+      int r = idx >> 5, c4 = (idx & 31) * 4;
+      long long row = row0 + r;
+      v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (row < a.R) {
+        const float *src = (const float *)a.src + row * a.rs;
+        int k = k0 + c4;
+        if (k + 3 < a.K && ((((uintptr_t)(src + k)) & 15) == 0)) {
+          v[it] = *(const float4 *)(src + k);
+        } else {
+          if (k < a.K) v[it].x = src[k];
+          if (k + 1 < a.K) v[it].y = src[k + 1];
+          if (k + 2 < a.K) v[it].z = src[k + 2];
+          if (k + 3 < a.K) v[it].w = src[k + 3];
+        }
       }
     }
-    float *t = tile + r * PITCH + c4;
-    t[0] = v.x; t[1] = v.y; t[2] = v.z; t[3] = v.w;
+  }
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    const int idx = tid + it * 256;
+    float *t = tile + (idx >> 5) * PITCH + (idx & 31) * 4;
+    t[0] = v[it].x; t[1] = v[it].y; t[2] = v[it].z; t[3] = v[it].w;
   }
   __syncthreads();
   constexpr int KF = MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32;
@@ -273,26 +291,42 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
   const long long row0 = rb * TR;
   const int k0 = kt * TK;
   // 32 k x 128 rows = 1024 float4 along rows, 4 per thread
+  float4 v[4];
+  const float *base = (const float *)a.src + (long long)k0 * a.ks + row0;
+  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.ks * 4)) & 15) == 0) {
+    // interior tile, 16-byte aligned: every load in flight at once
 #pragma unroll
-  for (int it = 0; it < 4; it++) {
-    int idx = tid + it * 256;
-    int kk = idx >> 5, r4 = (idx & 31) * 4;
-    int k = k0 + kk;
-    float v[4] = {0.f, 0.f, 0.f, 0.f};
-    if (k < a.K) {
-      const float *src = (const float *)a.src + (long long)k * a.ks + row0 + r4;
-      long long row = row0 + r4;
-      if (row + 3 < a.R && ((((uintptr_t)src) & 15) == 0)) {
-        float4 q = *(const float4 *)src;
-        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-      } else {
+    for (int it = 0; it < 4; it++) {
+      const int idx = tid + it * 256;
+      v[it] = *(const float4 *)(base + (long long)(idx >> 5) * a.ks + (idx & 31) * 4);
+    }
+  } else {
 #pragma unroll
-        for (int e = 0; e < 4; e++)
-          if (row + e < a.R) v[e] = src[e];
+    for (int it = 0; it < 4; it++) {
+      int idx = tid + it * 256;
+      int kk = idx >> 5, r4 = (idx & 31) * 4;
+      int k = k0 + kk;
+      v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (k < a.K) {
+        // (volatile keeps the compiler from merging these edge loads with
+        // the interior tile's vector loads)
+        const volatile float *src = (const float *)a.src + (long long)k * a.ks + row0 + r4;
+        long long row = row0 + r4;
+        if (row < a.R) v[it].x = src[0];
+        if (row + 1 < a.R) v[it].y = src[1];
+        if (row + 2 < a.R) v[it].z = src[2];
+        if (row + 3 < a.R) v[it].w = src[3];
       }
     }
+  }
 #pragma unroll
-    for (int e = 0; e < 4; e++) tile[(r4 + e) * PITCH + kk] = v[e];
+  for (int it = 0; it < 4; it++) {
+    const int idx = tid + it * 256;
+    const int kk = idx >> 5, r4 = (idx & 31) * 4;
+    tile[(r4 + 0) * PITCH + kk] = v[it].x;
+    tile[(r4 + 1) * PITCH + kk] = v[it].y;
+    tile[(r4 + 2) * PITCH + kk] = v[it].z;
+    tile[(r4 + 3) * PITCH + kk] = v[it].w;
   }
   __syncthreads();
   constexpr int E = KF / 2;
@@ -366,8 +400,11 @@ __global__ void k_pack_f64(PackArgs a) {
 // the word is read first (from L2) and the atomic issued only when it would
 // set a new bit; a stale read only costs a redundant atomic.
 __device__ inline void flags_or(int *flags, int mask) {
-  const int cur = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (mask & ~cur) atomicOr(flags, mask);
+  // this block's slot; the load first, so that once the bits are set the
+  // remaining waves only read
+  int *slot = flags + (blockIdx.x % NUMC_FLAG_SLOTS) * NUMC_FLAG_STRIDE;
+  const int cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (mask & ~cur) atomicOr(slot, mask);
 }
 
 template <class S, int P>
@@ -575,7 +612,9 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   // plane products: product q uses plane (pa0 + q) of A and (pb0 - q) of B
   int nprod = g.nprod, pa0 = g.pa[0], pb0 = g.pb[0];
   if (g.plane_flags) {
-    nprod = int_products(g.shift_pass, g.plane_flags[0], g.plane_flags[1], &pa0, &pb0);
+    int ma, mb;
+    plane_masks(g.plane_flags, &ma, &mb);
+    nprod = int_products(g.shift_pass, ma, mb, &pa0, &pb0);
     if (nprod == 0 && g.shift_pass > 0) return; // this digit pass adds nothing
   }
   const int V = nprod * g.nKC;
@@ -741,22 +780,25 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     WAIT_VMCNT(0);
   }
 
-  // epilogue
+  // epilogue (split-K slices store raw partial sums; their reduction stores)
   const int row0 = bm * BM + wm * WM, col0 = bn * BN + wn * WN;
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
-    for (int j = 0; j < TN; j++)
+    for (int j = 0; j < TN; j++) {
+      const int col = col0 + j * T + MF::col(lane);
+      const auto bias = g.ksplit == 1 ? epi_bias<EPI>(g, col) : (typename EpiBias<EPI>::T)0; // per lane
 #pragma unroll
       for (int v = 0; v < MF::NACC; v++) {
-        const int row = row0 + i * T + MF::row(v, lane), col = col0 + j * T + MF::col(lane);
+        const int row = row0 + i * T + MF::row(v, lane);
         if (row < g.M && col < g.N) {
           if (g.ksplit > 1)
             ((Raw *)g.partial)[((long long)slice * g.M + row) * g.N + col] = (Raw)acc[i][j][v];
           else
-            epi_store<EPI, Raw>(g, row, col, (Raw)acc[i][j][v]);
+            epi_store<EPI, Raw>(g, row, col, (Raw)acc[i][j][v], bias);
         }
       }
+    }
 }
 
 template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int VAR>
@@ -782,14 +824,17 @@ template <class Raw, int EPI>
 __global__ void k_splitk_reduce(const GemmArgs g) {
   if (g.plane_flags && g.shift_pass > 0) {
     int pa0, pb0;
-    if (int_products(g.shift_pass, g.plane_flags[0], g.plane_flags[1], &pa0, &pb0) == 0) return;
+    int ma, mb;
+    plane_masks(g.plane_flags, &ma, &mb);
+    if (int_products(g.shift_pass, ma, mb, &pa0, &pb0) == 0) return;
   }
   long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t >= (long long)g.M * g.N) return;
   const Raw *p = (const Raw *)g.partial;
   Raw s = p[t];
   for (int z = 1; z < g.ksplit; z++) s += p[(long long)z * g.M * g.N + t];
-  epi_store<EPI, Raw>(g, (int)(t / g.N), (int)(t % g.N), s);
+  const int col = (int)(t % g.N);
+  epi_store<EPI, Raw>(g, (int)(t / g.N), col, s, epi_bias<EPI>(g, col));
 }
 
 // ===========================================================================
@@ -819,7 +864,8 @@ __global__ void k_naive(const T *a, const T *b, T *c, int M, int K, int N) {
 // 64-bit integers: no MFMA; 16x16 output tile per block, k staged in LDS.
 template <class T>
 __global__ void k_gemm_i64(const T *a, const T *b, T *c, int M, int K, int N, long long rsa, long long csa,
-                           long long rsb, long long csb, long long rso, long long cso) {
+                           long long rsb, long long csb, long long rso, long long cso, const T *bias,
+                           long long bias_s) {
   __shared__ unsigned long long sa[16][17], sb[16][17];
   int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
@@ -833,7 +879,10 @@ __global__ void k_gemm_i64(const T *a, const T *b, T *c, int M, int K, int N, lo
     for (int k = 0; k < 16; k++) acc += sa[ty][k] * sb[k][tx];
     __syncthreads();
   }
-  if (row < M && col < N) c[row * rso + col * cso] = (T)acc;
+  if (row < M && col < N) {
+    if (bias) acc += (unsigned long long)bias[col * bias_s];
+    c[row * rso + col * cso] = (T)acc;
+  }
 }
 
 // ===========================================================================
@@ -908,20 +957,23 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
 }
 
 #define CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, VAR) \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>}
+  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>, nullptr}
 #define CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS) CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, 0)
-#define CFG(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT) CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, 2)
+// plain FP32 / FP64 / integer configurations also compile a biased-store twin (EPI + 4)
+#define CFGB(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                      \
+  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>}
 
 // (family, tile, k-chunk, waves, split-K, LDS stages); stages x chunk bytes <= 160 KiB
 static const GemmConfig kF32[] = {
-    CFGS(MfF32, 256, 256, 32, 2, 4, MODE_SINGLE, EPI_F32, 1, 2),
-    CFGS(MfF32, 256, 256, 16, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
-    CFGS(MfF32, 256, 128, 16, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
-    CFGS(MfF32, 128, 256, 16, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
-    CFGS(MfF32, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
-    CFGS(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
-    CFGS(MfF32, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F32, 4, 4),
-    CFGS(MfF32, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F32, 8, 4),
+    CFGB(MfF32, 256, 256, 32, 2, 4, EPI_F32, 1, 2),
+    CFGB(MfF32, 256, 256, 16, 2, 2, EPI_F32, 1, 4),
+    CFGB(MfF32, 256, 128, 16, 2, 2, EPI_F32, 1, 4),
+    CFGB(MfF32, 128, 256, 16, 2, 2, EPI_F32, 1, 4),
+    CFGB(MfF32, 128, 128, 32, 2, 2, EPI_F32, 1, 4),
+    CFGB(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 4),
+    CFGB(MfF32, 128, 128, 32, 2, 2, EPI_F32, 4, 4),
+    CFGB(MfF32, 64, 64, 32, 2, 2, EPI_F32, 8, 4),
 };
 static const GemmConfig kSplit[] = {
     CFGP(MfBF16, 256, 256, 32, 4, 2, MODE_SPLIT3, EPI_F32, 1, 2, 1),
@@ -944,22 +996,22 @@ static const GemmConfig kFP8[] = {
     CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 4, 4),
 };
 static const GemmConfig kF64[] = {
-    CFGS(MfF64, 128, 128, 16, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
-    CFGS(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
-    CFGS(MfF64, 128, 64, 16, 2, 2, MODE_SINGLE, EPI_F64, 1, 4),
-    CFGS(MfF64, 64, 64, 32, 2, 2, MODE_SINGLE, EPI_F64, 8, 4),
-    CFGS(MfF64, 256, 128, 16, 4, 2, MODE_SINGLE, EPI_F64, 1, 3),
-    CFGS(MfF64, 128, 128, 32, 2, 2, MODE_SINGLE, EPI_F64, 1, 2),
+    CFGB(MfF64, 128, 128, 16, 2, 2, EPI_F64, 1, 4),
+    CFGB(MfF64, 64, 64, 32, 2, 2, EPI_F64, 1, 4),
+    CFGB(MfF64, 128, 64, 16, 2, 2, EPI_F64, 1, 4),
+    CFGB(MfF64, 64, 64, 32, 2, 2, EPI_F64, 8, 4),
+    CFGB(MfF64, 256, 128, 16, 4, 2, EPI_F64, 1, 3),
+    CFGB(MfF64, 128, 128, 32, 2, 2, EPI_F64, 1, 2),
 };
 static const GemmConfig kI8[] = {
-    CFGS(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
-    CFGS(MfI8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
-    CFGS(MfI8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_INT, 1, 4),
-    CFGS(MfI8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_INT, 4, 4),
-    CFGS(MfI8, 256, 256, 64, 4, 2, MODE_SINGLE, EPI_INT, 1, 4),
-    CFGS(MfI8, 256, 256, 128, 4, 2, MODE_SINGLE, EPI_INT, 1, 2),
+    CFGB(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 4),
+    CFGB(MfI8, 256, 128, 64, 2, 2, EPI_INT, 1, 4),
+    CFGB(MfI8, 64, 64, 128, 2, 2, EPI_INT, 1, 4),
+    CFGB(MfI8, 128, 128, 64, 2, 2, EPI_INT, 4, 4),
+    CFGB(MfI8, 256, 256, 64, 4, 2, EPI_INT, 1, 4),
+    CFGB(MfI8, 256, 256, 128, 4, 2, EPI_INT, 1, 2),
 };
-#undef CFG
+#undef CFGB
 
 struct FamilyTable {
   const GemmConfig *cfg;
@@ -1055,18 +1107,27 @@ __device__ __attribute__((always_inline)) inline void digits_unit(const PackArgs
 
 // one 16-byte chunk of source values (EPC of them) into LDS as raw 32-bit
 // words; (S)word gives the value back
+// 16 bytes (16 / sizeof(S) values) of an edge chunk, zero past n_ok values;
+// volatile keeps the compiler from merging these scalar loads with the
+// interior tiles' vector loads
 template <class S>
-__device__ __attribute__((always_inline)) inline void load_chunk(const S *src, long long n_ok, uint32_t *v) {
+__device__ __attribute__((always_inline)) inline uint4 load_chunk_edge(const S *src, long long n_ok) {
   constexpr int EPC = 16 / sizeof(S);
-  if (n_ok >= EPC && ((((uintptr_t)src) & 15) == 0)) {
-    union { uint4 q; S e[EPC]; } u;
-    u.q = *(const uint4 *)src;
+  union { uint4 q; S e[EPC]; } u;
+  const volatile S *vs = src;
 #pragma unroll
-    for (int e = 0; e < EPC; e++) v[e] = (uint32_t)u.e[e];
-  } else {
+  for (int e = 0; e < EPC; e++) u.e[e] = e < n_ok ? vs[e] : (S)0;
+  return u.q;
+}
+
+// the values of a 16-byte chunk as 32-bit words; (S)word gives each back
+template <class S>
+__device__ __attribute__((always_inline)) inline void unpack_chunk(uint4 q, uint32_t *v) {
+  constexpr int EPC = 16 / sizeof(S);
+  union { uint4 q; S e[EPC]; } u;
+  u.q = q;
 #pragma unroll
-    for (int e = 0; e < EPC; e++) v[e] = e < n_ok ? (uint32_t)src[e] : 0u;
-  }
+  for (int e = 0; e < EPC; e++) v[e] = (uint32_t)u.e[e];
 }
 
 // integer digit planes, rows with unit k-stride: 32 rows x 128 k tiles
@@ -1075,18 +1136,36 @@ template <class S, int P>
 __device__ __attribute__((always_inline)) inline void digits_tile_k(const PackArgs &a, long long bid, uint32_t *tile) {
   constexpr int TK = 128, PITCH = TK + 1, EPC = 16 / sizeof(S), CPR = TK / EPC;
   static_assert(32 * PITCH <= PACK_TILE_WORDS, "tile");
+  constexpr int NL = 32 * CPR / 256; // 16-byte loads per thread
+  static_assert(NL >= 1 && 32 * CPR % 256 == 0, "chunks");
   const int nkt = (a.nKF * 32 + TK - 1) / TK;
   const long long rt = bid / nkt;
   const int kt = (int)(bid % nkt);
   const int tid = threadIdx.x;
   const long long row0 = rt * 32;
   const int k0 = kt * TK;
-  for (int idx = tid; idx < 32 * CPR; idx += 256) {
-    const int r = idx / CPR, c = (idx % CPR) * EPC;
-    const long long row = row0 + r;
+  const S *base = (const S *)a.src + row0 * a.rs + k0;
+  uint4 q[NL];
+  if (row0 + 32 <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.rs * sizeof(S))) & 15) == 0) {
+    // interior tile, 16-byte aligned rows: every load in flight at once
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+      const int idx = tid + it * 256;
+      q[it] = *(const uint4 *)(base + (idx / CPR) * a.rs + (idx % CPR) * EPC);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+      const int idx = tid + it * 256, r = idx / CPR, c = (idx % CPR) * EPC;
+      q[it] = make_uint4(0u, 0u, 0u, 0u);
+      if (row0 + r < a.R && k0 + c < a.K) q[it] = load_chunk_edge<S>(base + r * a.rs + c, (long long)a.K - (k0 + c));
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NL; it++) {
+    const int idx = tid + it * 256, r = idx / CPR, c = (idx % CPR) * EPC;
     uint32_t v[EPC];
-    const long long n_ok = row < a.R ? (long long)a.K - (k0 + c) : 0;
-    load_chunk<S>((const S *)a.src + (row < a.R ? row * a.rs + k0 + c : 0), n_ok, v);
+    unpack_chunk<S>(q[it], v);
 #pragma unroll
     for (int e = 0; e < EPC; e++) tile[r * PITCH + c + e] = v[e];
   }
@@ -1114,18 +1193,37 @@ __device__ __attribute__((always_inline)) inline void digits_tile_rows(const Pac
   // one k fragment (32 k) per tile; consecutive blocks take neighbouring
   // row blocks at one k range, so the segments they read complete each
   // other's source rows
+  constexpr int NL = TK * CPK / 256; // 16-byte loads per thread
+  static_assert(NL >= 1 && TK * CPK % 256 == 0, "chunks");
   const int nrb = (a.nRT + 3) / 4;
   const long long rb = bid % nrb;
   const int kt = (int)(bid / nrb);
   const int tid = threadIdx.x;
   const long long row0 = rb * TR;
   const int k0 = kt * TK;
-  for (int idx = tid; idx < TK * CPK; idx += 256) {
-    const int kk = idx / CPK, r = (idx % CPK) * EPC;
-    const int k = k0 + kk;
+  const S *base = (const S *)a.src + (long long)k0 * a.ks + row0;
+  uint4 q[NL];
+  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.ks * sizeof(S))) & 15) == 0) {
+    // interior tile, 16-byte aligned: every load in flight at once
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+      const int idx = tid + it * 256;
+      q[it] = *(const uint4 *)(base + (long long)(idx / CPK) * a.ks + (idx % CPK) * EPC);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+      const int idx = tid + it * 256, kk = idx / CPK, r = (idx % CPK) * EPC;
+      q[it] = make_uint4(0u, 0u, 0u, 0u);
+      if (k0 + kk < a.K && row0 + r < a.R)
+        q[it] = load_chunk_edge<S>(base + (long long)kk * a.ks + r, (long long)a.R - (row0 + r));
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NL; it++) {
+    const int idx = tid + it * 256, kk = idx / CPK, r = (idx % CPK) * EPC;
     uint32_t v[EPC];
-    const long long n_ok = k < a.K ? (long long)a.R - (row0 + r) : 0;
-    load_chunk<S>((const S *)a.src + (k < a.K ? (long long)k * a.ks + row0 + r : 0), n_ok, v);
+    unpack_chunk<S>(q[it], v);
 #pragma unroll
     for (int e = 0; e < EPC; e++) tile[(r + e) * PITCH + kk] = v[e];
   }
@@ -1143,6 +1241,148 @@ template <class S, int P>
 __global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   digits_tile_rows<S, P>(a, blockIdx.x, tile);
+}
+
+// ---------------------------------------------------------------------------
+// one-byte operands with one digit plane (int8 and uint8 results): a digit is
+// the byte itself, so a fragment lane is 16 source bytes, copied as is
+// ---------------------------------------------------------------------------
+
+// 16 source bytes at p of an edge tile, n_ok of them inside the matrix (zero past it)
+__device__ __attribute__((always_inline)) inline uint4 load16_bytes(const unsigned char *p, long long n_ok) {
+  return load_chunk_edge<unsigned char>(p, n_ok);
+}
+
+// k-contiguous rows: a 32-row x 512-k byte tile (16 fragments).  Each row
+// segment is read as 512 contiguous bytes, staged in LDS (row pitch 528 B,
+// so the 16-byte lane reads of consecutive rows fall on different banks),
+// then written out as 16 whole fragments.
+__device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArgs &a, long long bid, uint32_t *tile) {
+  constexpr int KB = 512, CPR = KB / 16, PW = CPR + 1, NF = KB / 32;
+  static_assert(32 * PW * 4 <= PACK_TILE_WORDS, "tile");
+  constexpr int NL = 32 * CPR / 256; // 16-byte loads per thread
+  uint4 *t = (uint4 *)tile;
+  const int nkb = (a.nKF + NF - 1) / NF;
+  const long long rt = bid / nkb;
+  const int kb = (int)(bid % nkb);
+  const int tid = threadIdx.x;
+  const unsigned char *base = (const unsigned char *)a.src + rt * 32 * a.rs + (long long)kb * KB;
+  uint4 v[NL];
+  if (rt * 32 + 32 <= a.R && kb * KB + KB <= a.K && ((((uintptr_t)base) | (uintptr_t)a.rs) & 15) == 0) {
+    // interior tile, 16-byte aligned rows: every load in flight at once
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+      const int idx = tid + it * 256;
+      v[it] = *(const uint4 *)(base + (idx / CPR) * a.rs + (idx % CPR) * 16);
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < NL; it++) {
+      const int idx = tid + it * 256, r = idx / CPR, c = idx % CPR;
+      const int k = kb * KB + c * 16;
+      v[it] = make_uint4(0u, 0u, 0u, 0u);
+      if (rt * 32 + r < a.R && k < a.K) v[it] = load16_bytes(base + r * a.rs + c * 16, (long long)a.K - k);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NL; it++) {
+    const int idx = tid + it * 256;
+    t[(idx / CPR) * PW + idx % CPR] = v[it];
+  }
+  __syncthreads();
+  bool any = false;
+#pragma unroll
+  for (int it = 0; it < NF * 64 / 256; it++) {
+    const int u = tid + it * 256, f = u >> 6, lane = u & 63;
+    const int kf = kb * NF + f;
+    if (kf < a.nKF) {
+      const uint4 v = t[(lane & 31) * PW + f * 2 + (lane >> 5)];
+      *(uint4 *)(a.dst + (rt * a.nKF + kf) * 1024 + lane * 16) = v;
+      any |= (v.x | v.y | v.z | v.w) != 0;
+    }
+  }
+  if (__ballot(any) && (threadIdx.x & 63) == 0 && a.flags) flags_or(a.flags, 1);
+}
+
+__global__ void __launch_bounds__(256) k_pack_bytes_k(PackArgs a) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  bytes_tile_k(a, blockIdx.x, tile);
+}
+
+// row-contiguous rows (B^T of a row-major B): a 128-row x 128-k byte tile.
+// Each thread reads 16 consecutive rows at 4 consecutive k (four 16-byte
+// loads), transposes the 4 x 16 bytes in registers and writes 16 dwords of
+// 4 k each into LDS; each fragment lane then reads its 16 k of one row.
+// The 16-byte column chunks of row r sit at chunk ^ ((r >> 4) & 7), so the
+// eight 16-row groups that one wave writes fall on different banks.
+__device__ __attribute__((always_inline)) inline void bytes_tile_rows(const PackArgs &a, long long bid, uint32_t *tile) {
+  constexpr int TR = 128, TK = 128, PITCH = TK + 16; // bytes per LDS row
+  static_assert(TR * PITCH / 4 <= PACK_TILE_WORDS, "tile");
+  unsigned char *t8 = (unsigned char *)tile;
+  const int nrb = (a.nRT + 3) / 4;
+  const long long rb = bid % nrb;
+  const int kt = (int)(bid / nrb);
+  const long long row0 = rb * TR;
+  const int k0 = kt * TK;
+  const int tid = threadIdx.x;
+  // thread -> (k quad, 16-row group): 32 quads x 8 groups
+  const int kq = tid >> 3, rg = tid & 7;
+  const int kk = k0 + kq * 4;
+  const long long r = row0 + rg * 16;
+  uint32_t w[4][4]; // [k offset][dword of 4 rows]
+  uint4 v[4];
+  const unsigned char *base = (const unsigned char *)a.src + (long long)k0 * a.ks + row0;
+  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)a.ks) & 15) == 0) {
+    // interior tile, 16-byte aligned: all four loads in flight at once
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = *(const uint4 *)(base + (long long)(kq * 4 + q) * a.ks + rg * 16);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = kk + q;
+      v[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (k < a.K && r < a.R) v[q] = load16_bytes((const unsigned char *)a.src + (long long)k * a.ks + r, a.R - r);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    w[q][0] = v[q].x;
+    w[q][1] = v[q].y;
+    w[q][2] = v[q].z;
+    w[q][3] = v[q].w;
+  }
+  // row rg*16 + 4g + j gets bytes j of w[0..3][g]: 4 consecutive k
+#pragma unroll
+  for (int g = 0; g < 4; g++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t d = ((w[0][g] >> (8 * j)) & 0xFFu) | (((w[1][g] >> (8 * j)) & 0xFFu) << 8) |
+                         (((w[2][g] >> (8 * j)) & 0xFFu) << 16) | (((w[3][g] >> (8 * j)) & 0xFFu) << 24);
+      *(uint32_t *)(t8 + (rg * 16 + 4 * g + j) * PITCH + (((kq >> 2) ^ rg) << 4) + (kq & 3) * 4) = d;
+    }
+  __syncthreads();
+  // 4 row fragments x 4 k fragments x 64 lanes: four lane slots per thread
+  bool any = false;
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    const int u = tid + it * 256;
+    const int f = u >> 6, lane = u & 63;
+    const int rl = f >> 2, fk = f & 3;
+    const long long rt = rb * 4 + rl;
+    const int kf = kt * 4 + fk;
+    if (rt < a.nRT && kf < a.nKF) {
+      const int row = rl * 32 + (lane & 31);
+      const uint4 v = *(const uint4 *)(t8 + row * PITCH + (((fk * 2 + (lane >> 5)) ^ ((row >> 4) & 7)) << 4));
+      *(uint4 *)(a.dst + (rt * a.nKF + kf) * 1024 + lane * 16) = v;
+      any |= (v.x | v.y | v.z | v.w) != 0;
+    }
+  }
+  if (__ballot(any) && (threadIdx.x & 63) == 0 && a.flags) flags_or(a.flags, 1);
+}
+
+__global__ void __launch_bounds__(256) k_pack_bytes_rows(PackArgs a) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  bytes_tile_rows(a, blockIdx.x, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,6 +1405,16 @@ __global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, con
   const PackArgs a = descs[lo];
   const long long u = b - prefix[lo];
   constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : 1;
+  if constexpr (FAM == NUMC_FAMILY_I8) {
+    if (a.mode == 3) {
+      bytes_tile_k(a, u, tile);
+      return;
+    }
+    if (a.mode == 4) {
+      bytes_tile_rows(a, u, tile);
+      return;
+    }
+  }
   if (a.mode == 0) {
     if constexpr (FAM == NUMC_FAMILY_I8) digits_tile_k<int8_t, 1>(a, u, tile);
     else pack_f32_tile_k<MODE_PK>(a, u, (float *)tile);
@@ -1184,6 +1434,14 @@ __global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, con
 // blocks the batched pack gives one operand, and the walk it takes
 int numc_batch_pack_blocks(int family, PackArgs &pk) {
   const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
+  if (family == NUMC_FAMILY_I8 && pk.ks == 1) { // int8 operands, one plane: byte tiles
+    pk.mode = 3;
+    return (int)(pk.nRT * cdiv(pk.nKF, 16));
+  }
+  if (family == NUMC_FAMILY_I8 && pk.rs == 1) {
+    pk.mode = 4;
+    return (int)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4));
+  }
   if (pk.ks == 1) {
     pk.mode = 0;
     return (int)((long long)pk.nRT * cdiv((long long)pk.nKF * kf, 128));
@@ -1197,6 +1455,17 @@ int numc_batch_pack_blocks(int family, PackArgs &pk) {
 }
 
 template <class S, int P> static void launch_digits_p(PackArgs &pk, unsigned nb, hipStream_t st) {
+  if constexpr (sizeof(S) == 1 && P == 1) {
+    if (pk.ks == 1) {
+      hipLaunchKernelGGL(k_pack_bytes_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16))), dim3(256), 0, st, pk);
+      return;
+    }
+    if (pk.rs == 1) {
+      dim3 grid((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4)));
+      hipLaunchKernelGGL(k_pack_bytes_rows, dim3(grid), dim3(256), 0, st, pk);
+      return;
+    }
+  }
   if (pk.ks == 1) {
     dim3 grid((unsigned)((long long)pk.nRT * cdiv((long long)pk.nKF * 32, 128)));
     hipLaunchKernelGGL((k_pack_digits_tiled<S, P>), grid, dim3(256), 0, st, pk);
@@ -1286,7 +1555,7 @@ size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *au
   size_t raw = (p.family == NUMC_FAMILY_F64) ? 8 : 4;
   size_t partial = c.cfg.split_k > 1 ? (size_t)c.cfg.split_k * p.M * p.N * raw : 0;
   size_t bytes = (size_t)(pa + pb) * planes + partial + 3 * 256;
-  if (aux_bytes) *aux_bytes = 256 + (size_t)(p.M + p.N) * 8 + 512;
+  if (aux_bytes) *aux_bytes = 2 * NUMC_FLAG_INTS * sizeof(int) + (size_t)(p.M + p.N) * 8 + 512;
   return bytes;
 }
 
@@ -1296,11 +1565,13 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   const int kf = family_kf(family);
   const int planes = family == NUMC_FAMILY_SPLIT_BF16 ? 2 : (family == NUMC_FAMILY_I8 ? digit_planes(p.out_dt) : 1);
 
+  if (p.bias && (family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_FP8))
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: a bias needs the plain method (family %d)", family);
   if (p.M == 0 || p.N == 0) return 0;
   if (p.K == 0) {
     long long n = (long long)p.M * p.N;
     hipLaunchKernelGGL(k_zero_strided, dim3(blocks_for(n)), dim3(256), 0, st, (unsigned char *)p.C, p.M, p.N,
-                       p.rso, p.cso, (int)numc_dtype_size(p.out_dt));
+                       p.rso, p.cso, (int)numc_dtype_size(p.out_dt), (const unsigned char *)p.bias, p.bias_s);
     return 0;
   }
 
@@ -1315,9 +1586,10 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   void *partial = c.cfg.split_k > 1 ? (void *)(Bp + plb * planes) : nullptr;
   (void)raw;
 
-  // aux: [0..1] digit plane masks, then FP8 row max bits / exponents
+  // aux: the digit plane masks (NUMC_FLAG_INTS ints per operand), then FP8
+  // row max bits / exponents
   int *flags = (int *)aux;
-  uint32_t *bitsA = (uint32_t *)((unsigned char *)aux + 256);
+  uint32_t *bitsA = (uint32_t *)((unsigned char *)aux + 2 * NUMC_FLAG_INTS * sizeof(int));
   uint32_t *bitsB = bitsA + p.M;
   int *expA = (int *)(bitsB + p.N);
   int *expB = expA + p.M;
@@ -1330,7 +1602,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     hipLaunchKernelGGL(k_exps, dim3(blocks_for(p.M + p.N)), dim3(256), 0, st, bitsA, expA, p.M + p.N);
   }
   if (family == NUMC_FAMILY_I8)
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)flags, 2LL, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * NUMC_FLAG_INTS)), dim3(256), 0, st, (uint32_t *)flags,
+                       (long long)(2 * NUMC_FLAG_INTS), 0u);
 
   // A: rows M, element (i, k) at A[i*rsa + k*csa]
   pk.src = p.A; pk.rs = p.rsa; pk.ks = p.csa; pk.R = p.M; pk.K = p.K;
@@ -1341,7 +1614,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // B^T: rows N, element (j, k) at B[k*rsb + j*csb]
   pk.src = p.B; pk.rs = p.csb; pk.ks = p.rsb; pk.R = p.N; pk.K = p.K;
   pk.nRT = nRTb; pk.nKF = nKF; pk.dst = Bp; pk.plane_bytes = plb;
-  pk.row_exp = expB; pk.flags = flags + 1;
+  pk.row_exp = expB; pk.flags = flags + NUMC_FLAG_INTS;
   launch_pack(family, p.in_dt, planes, pk, st);
   numc_prof_end(ctx);
 
@@ -1366,21 +1639,33 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.partial = partial;
   g.nprod = 1;
   g.pa[0] = g.pb[0] = 0;
+  g.bias = p.bias;
+  g.bias_s = p.bias_s;
   dim3 grid((unsigned)(g.nbm * g.nbn), (unsigned)g.ksplit);
+  const bool bias = p.bias != nullptr;
   const int passes = family == NUMC_FAMILY_I8 ? planes : 1;
   for (int s = 0; s < passes; s++) {
     g.shift_pass = s;
     g.plane_flags = family == NUMC_FAMILY_I8 ? flags : nullptr;
     numc_prof_begin(ctx, PROF_GEMM, family);
-    c.launch(g, grid, st);
+    (bias && c.launch_bias ? c.launch_bias : c.launch)(g, grid, st);
     numc_prof_end(ctx);
     if (g.ksplit > 1) {
       unsigned nb = blocks_for((long long)p.M * p.N);
       switch (family) {
-      case NUMC_FAMILY_F64: hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64>), dim3(nb), dim3(256), 0, st, g); break;
+      case NUMC_FAMILY_F64:
+        if (bias) hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64B>), dim3(nb), dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64>), dim3(nb), dim3(256), 0, st, g);
+        break;
       case NUMC_FAMILY_FP8: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_FP8>), dim3(nb), dim3(256), 0, st, g); break;
-      case NUMC_FAMILY_I8: hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INT>), dim3(nb), dim3(256), 0, st, g); break;
-      default: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32>), dim3(nb), dim3(256), 0, st, g); break;
+      case NUMC_FAMILY_I8:
+        if (bias) hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INTB>), dim3(nb), dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INT>), dim3(nb), dim3(256), 0, st, g);
+        break;
+      default:
+        if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32B>), dim3(nb), dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32>), dim3(nb), dim3(256), 0, st, g);
+        break;
       }
     }
   }
@@ -1463,10 +1748,12 @@ int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p) {
   dim3 grid((unsigned)cdiv(p.N, 16), (unsigned)cdiv(p.M, 16));
   if (p.in_dt == NUMC_DTYPE_INT64)
     hipLaunchKernelGGL((k_gemm_i64<int64_t>), grid, dim3(256), 0, ctx->stream, (const int64_t *)p.A,
-                       (const int64_t *)p.B, (int64_t *)p.C, p.M, p.K, p.N, p.rsa, p.csa, p.rsb, p.csb, p.rso, p.cso);
+                       (const int64_t *)p.B, (int64_t *)p.C, p.M, p.K, p.N, p.rsa, p.csa, p.rsb, p.csb, p.rso, p.cso,
+                       (const int64_t *)p.bias, p.bias_s);
   else
     hipLaunchKernelGGL((k_gemm_i64<uint64_t>), grid, dim3(256), 0, ctx->stream, (const uint64_t *)p.A,
-                       (const uint64_t *)p.B, (uint64_t *)p.C, p.M, p.K, p.N, p.rsa, p.csa, p.rsb, p.csb, p.rso, p.cso);
+                       (const uint64_t *)p.B, (uint64_t *)p.C, p.M, p.K, p.N, p.rsa, p.csa, p.rsb, p.csb, p.rso, p.cso,
+                       (const uint64_t *)p.bias, p.bias_s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "i64 launch");
   return 0;

@@ -104,7 +104,10 @@ struct MfF64 { // v_mfma_f64_16x16x4_f64 x2 per fragment
 // ---------------------------------------------------------------------------
 
 enum GemmMode { MODE_SINGLE = 0, MODE_SPLIT3 = 1 };
-enum GemmEpi { EPI_F32 = 0, EPI_F64 = 1, EPI_FP8 = 2, EPI_INT = 3 };
+// EPI_F32B / EPI_F64B / EPI_INTB: the same outputs plus a per-column bias
+// (numc_linear), separate kernels so that the plain stores keep their code
+// and register budget
+enum GemmEpi { EPI_F32 = 0, EPI_F64 = 1, EPI_FP8 = 2, EPI_INT = 3, EPI_F32B = 4, EPI_F64B = 5, EPI_INTB = 7 };
 
 struct GemmArgs {
   const unsigned char *A, *B; // packed operands
@@ -124,7 +127,34 @@ struct GemmArgs {
   void *partial;              // split-K partial sums [ksplit][M][N] (raw accumulator type)
   int nprod;                  // MODE_SINGLE: products concatenated along K
   int pa[4], pb[4];           // plane of A / B for each product
+  const void *bias;           // optional per-column bias (output dtype), added at the store
+  long long bias_s;           // bias stride in elements
 };
+
+// Nonzero-digit-plane masks of the integer packs: each operand's mask is
+// spread over NUMC_FLAG_SLOTS words, one per 128-byte line, so that the
+// pack's blocks OR into different L2 lines instead of queueing on one
+// address; a reader ORs the slots.
+#define NUMC_FLAG_SLOTS 32
+#define NUMC_FLAG_STRIDE 32                                 // ints between slots
+#define NUMC_FLAG_INTS (NUMC_FLAG_SLOTS * NUMC_FLAG_STRIDE) // ints per operand
+
+// Both operands' masks in one vector load: lanes 0-31 read A's slots, lanes
+// 32-63 B's, and a ballot per plane bit ORs them (call with the whole wave).
+static_assert(NUMC_FLAG_SLOTS == 32, "one slot per lane of a half wave");
+__device__ inline void plane_masks(const int *flags, int *ma, int *mb) {
+  const int lane = threadIdx.x & 63;
+  const int v = flags[(lane >> 5) * NUMC_FLAG_INTS + (lane & 31) * NUMC_FLAG_STRIDE];
+  int a = 0, b = 0;
+#pragma unroll
+  for (int p = 0; p < 4; p++) { // at most 4 digit planes
+    const unsigned long long bal = __ballot((v >> p) & 1);
+    if (bal & 0xFFFFFFFFull) a |= 1 << p;
+    if (bal >> 32) b |= 1 << p;
+  }
+  *ma = a;
+  *mb = b;
+}
 
 // Products of integer digit pass s: (i, s - i) for i = pa0 .. pa0 + n - 1,
 // restricted to planes holding a nonzero digit (masks from the pack kernels).
@@ -140,17 +170,48 @@ __device__ inline int int_products(int s, int maskA, int maskB, int *pa0, int *p
 // epilogue helpers
 // ---------------------------------------------------------------------------
 
+// The bias term of column col as the store adds it (0 without a bias and
+// for epilogues that take none).  Integer outputs: the low 32 bits, added by
+// the first digit pass only.  Split-K slices store raw partial sums and their
+// reduction adds the bias.
+template <int EPI> struct EpiBias { using T = uint32_t; };
+template <> struct EpiBias<EPI_F32B> { using T = float; };
+template <> struct EpiBias<EPI_F64B> { using T = double; };
+
+template <int EPI>
+__device__ inline typename EpiBias<EPI>::T epi_bias(const GemmArgs &g, int col) {
+  using T = typename EpiBias<EPI>::T;
+  if constexpr (EPI == EPI_F32B || EPI == EPI_F64B) {
+    return col < g.N ? ((const T *)g.bias)[(long long)col * g.bias_s] : (T)0;
+  } else if constexpr (EPI == EPI_INTB) {
+    if (g.shift_pass || col >= g.N) return 0u;
+    const long long b = (long long)col * g.bias_s;
+    return g.out_bytes == 1 ? (uint32_t)((const uint8_t *)g.bias)[b]
+           : g.out_bytes == 2 ? (uint32_t)((const uint16_t *)g.bias)[b]
+                              : ((const uint32_t *)g.bias)[b];
+  } else {
+    return 0u;
+  }
+}
+
 template <int EPI, class Raw>
-__device__ inline void epi_store(const GemmArgs &g, int row, int col, Raw v) {
+__device__ inline void epi_store(const GemmArgs &g, int row, int col, Raw v, typename EpiBias<EPI>::T bias) {
   const long long off = (long long)row * g.rso + (long long)col * g.cso;
   if constexpr (EPI == EPI_F32) {
     ((float *)g.C)[off] = (float)v;
+  } else if constexpr (EPI == EPI_F32B) {
+    ((float *)g.C)[off] = (float)v + bias;
   } else if constexpr (EPI == EPI_F64) {
     ((double *)g.C)[off] = (double)v;
+  } else if constexpr (EPI == EPI_F64B) {
+    ((double *)g.C)[off] = (double)v + bias;
   } else if constexpr (EPI == EPI_FP8) {
     ((float *)g.C)[off] = ldexpf((float)v, -(g.row_exp[row] + g.col_exp[col]));
   } else {
+    // integer digit pass s adds (sum << 8s); the first pass writes the output,
+    // with the bias (mod 2^w) under EPI_INTB
     uint32_t x = (uint32_t)(int)v << (8 * g.shift_pass);
+    if constexpr (EPI == EPI_INTB) x += bias;
     if (g.out_bytes == 1) {
       uint8_t *p = (uint8_t *)g.C + off;
       *p = (uint8_t)(g.shift_pass ? (uint32_t)*p + x : x);

@@ -165,8 +165,10 @@ static int launch_problem(NumcCtx *ctx, const GemmProblem &p, const NumcMatmulOp
   return numc_gemm_launch(ctx, p, *cfg, ctx->ws.ptr, ctx->ws_aux.ptr);
 }
 
-int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts,
-                    bool finish) {
+// one 2-D problem out = a @ b (+ bias[j] on every row when bias is given, a
+// 1-D array of the output dtype, added in the GEMM's store)
+static int matmul_bias(const NumcArray *a, const NumcArray *b, const NumcArray *bias, NumcArray *out,
+                       const NumcMatmulOpts *opts, bool finish) {
   const bool widen = opts && (opts->flags & NUMC_OPT_WIDEN);
   int err = check_matmul(a, b, out, widen);
   if (err) return err;
@@ -193,11 +195,20 @@ int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, cons
 
   if ((err = stage_in(ctx, a, &p.A))) return err;
   if ((err = stage_in(ctx, b, &p.B))) return err;
+  if (bias) {
+    if ((err = stage_in(ctx, bias, &p.bias))) return err;
+    p.bias_s = es_stride(bias->strides[0], eout);
+  }
   if ((err = stage_out_begin(ctx, out, &p.C))) return err;
 
   if ((err = launch_problem(ctx, p, opts))) return err;
   if ((err = stage_out_end(ctx, out, p.C))) return err;
   return finish ? numc_finish(ctx) : 0;
+}
+
+int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts,
+                    bool finish) {
+  return matmul_bias(a, b, nullptr, out, opts, finish);
 }
 
 static int run_matmul(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts) {
@@ -547,4 +558,153 @@ extern "C" NUMC_API int numc_dot(const NumcArray *a, const NumcArray *b, NumcArr
   if (err) return err;
   if (a->dim == 0 || b->dim == 0) return dot_scalar(a, b, out);
   return dot_gemm(a, b, out);
+}
+
+// ===========================================================================
+// numc_linear and numc_bmm: the GEMM consumers numc's roadmap names
+// (ROADMAP.md:139-140).  numc has no implementation of either, so the
+// validation below follows numc_matmul's order (NULL -> dtype -> ndim ->
+// inner dimension -> output shape) and message style.
+// ===========================================================================
+
+static int check_linear(const NumcArray *x, const NumcArray *w, const NumcArray *bias, const NumcArray *out) {
+  if (!x || !w || !out)
+    return NUMC_SET_ERROR(NUMC_ERR_NULL, "linear: NULL pointer (x=%p w=%p out=%p)", (const void *)x,
+                          (const void *)w, (const void *)out);
+  if (x->dtype != w->dtype || x->dtype != out->dtype || (bias && bias->dtype != out->dtype))
+    return NUMC_SET_ERROR(NUMC_ERR_TYPE, "linear: dtype mismatch (x=%d w=%d bias=%d out=%d)", x->dtype, w->dtype,
+                          bias ? (int)bias->dtype : -1, out->dtype);
+  if (x->dim < 1 || w->dim != 2 || out->dim != x->dim || (bias && bias->dim != 1))
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE,
+                          "linear: ndim mismatch (x.dim=%zu w.dim=%zu bias.dim=%zu out.dim=%zu) expected x "
+                          "(..., K), w (K, N), bias (N), out (..., N)",
+                          x->dim, w->dim, bias ? bias->dim : (size_t)1, out->dim);
+  const size_t K = x->shape[x->dim - 1], N = w->shape[1];
+  if (w->shape[0] != K)
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "linear: shape mismatch (x.shape[-1]=%zu w.shape[0]=%zu)", K,
+                          w->shape[0]);
+  if (bias && bias->shape[0] != N)
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "linear: shape mismatch (bias.shape[0]=%zu w.shape[1]=%zu)",
+                          bias->shape[0], N);
+  for (size_t i = 0; i + 1 < x->dim; i++)
+    if (out->shape[i] != x->shape[i])
+      return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "linear: shape mismatch (out.shape[%zu]=%zu x.shape[%zu]=%zu)", i,
+                            out->shape[i], i, x->shape[i]);
+  if (out->shape[out->dim - 1] != N)
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "linear: shape mismatch (out.shape[-1]=%zu w.shape[1]=%zu)",
+                          out->shape[out->dim - 1], N);
+  return 0;
+}
+
+extern "C" NUMC_API int numc_linear_ex(const NumcArray *x, const NumcArray *w, const NumcArray *bias, NumcArray *out,
+                                       const NumcMatmulOpts *opts) {
+  int err = check_linear(x, w, bias, out);
+  if (err) return err;
+  if (opts && (opts->method != NUMC_METHOD_AUTO && opts->method != NUMC_METHOD_PLAIN))
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "linear: method %d has no biased store (use AUTO or PLAIN)", opts->method);
+  if (opts && (opts->flags & NUMC_OPT_WIDEN))
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "linear: NUMC_OPT_WIDEN is not supported");
+  if (out->size == 0) return 0;
+  const size_t es = x->elem_size;
+  const size_t K = x->shape[x->dim - 1], N = w->shape[1];
+  const size_t na = x->dim - 1; // leading axes of x (and of out)
+  const size_t csa = x->strides[x->dim - 1], cso = out->strides[out->dim - 1];
+  // the trailing run of leading axes that is linear in x and in out folds into M
+  size_t c = na;
+  if (na > 0) {
+    c = na - 1;
+    while (c > 0 && x->strides[c - 1] == x->strides[c] * x->shape[c] &&
+           out->strides[c - 1] == out->strides[c] * out->shape[c])
+      c--;
+  }
+  size_t M = 1, n_outer = 1;
+  for (size_t i = c; i < na; i++) M *= x->shape[i];
+  for (size_t i = 0; i < c; i++) n_outer *= x->shape[i];
+  const size_t rsa = na > 0 ? x->strides[na - 1] : K * es;
+  const size_t rso = na > 0 ? out->strides[na - 1] : N * out->elem_size;
+  NumcArray va, vo;
+  for (size_t t = 0; t < n_outer; t++) {
+    size_t oi = t, aoff = 0, ooff = 0;
+    for (size_t i = c; i-- > 0;) {
+      const size_t idx = oi % x->shape[i];
+      oi /= x->shape[i];
+      aoff += idx * x->strides[i];
+      ooff += idx * out->strides[i];
+    }
+    view2(&va, x, aoff, M, K, rsa, csa);
+    view2(&vo, out, ooff, M, N, rso, cso);
+    if ((err = matmul_bias(&va, w, bias, &vo, opts, false))) return err;
+  }
+  return numc_finish(out->ctx);
+}
+
+extern "C" NUMC_API int numc_linear(const NumcArray *x, const NumcArray *w, const NumcArray *bias, NumcArray *out) {
+  return numc_linear_ex(x, w, bias, out, nullptr);
+}
+
+static int check_bmm(const NumcArray *a, const NumcArray *b, const NumcArray *out) {
+  if (!a || !b || !out)
+    return NUMC_SET_ERROR(NUMC_ERR_NULL, "bmm: NULL pointer (a=%p b=%p out=%p)", (const void *)a, (const void *)b,
+                          (const void *)out);
+  if (a->dtype != b->dtype || a->dtype != out->dtype)
+    return NUMC_SET_ERROR(NUMC_ERR_TYPE, "bmm: dtype mismatch (a=%d b=%d out=%d)", a->dtype, b->dtype, out->dtype);
+  const size_t nd = a->dim > b->dim ? a->dim : b->dim;
+  if (a->dim < 2 || b->dim < 2 || out->dim != nd)
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE,
+                          "bmm: ndim mismatch (a.dim=%zu b.dim=%zu out.dim=%zu) expected a (..., M, K), b (..., K, "
+                          "N), out (..., M, N) with the broadcast batch axes",
+                          a->dim, b->dim, out->dim);
+  if (a->shape[a->dim - 1] != b->shape[b->dim - 2])
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "bmm: shape mismatch (a.shape[-1]=%zu b.shape[-2]=%zu)",
+                          a->shape[a->dim - 1], b->shape[b->dim - 2]);
+  // batch axes, aligned from the right: equal, or 1 in one operand (broadcast)
+  for (size_t i = 0; i + 2 < nd; i++) {
+    const size_t ia = i + a->dim, ib = i + b->dim; // index + nd - offset, kept unsigned
+    const size_t da = ia >= nd ? a->shape[ia - nd] : 1, db = ib >= nd ? b->shape[ib - nd] : 1;
+    if (da != db && da != 1 && db != 1)
+      return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "bmm: batch axis %zu does not broadcast (a=%zu b=%zu)", i, da, db);
+    if (out->shape[i] != (da == 1 ? db : da))
+      return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "bmm: shape mismatch (out.shape[%zu]=%zu expected %zu)", i,
+                            out->shape[i], da == 1 ? db : da);
+  }
+  if (out->shape[nd - 2] != a->shape[a->dim - 2] || out->shape[nd - 1] != b->shape[b->dim - 1])
+    return NUMC_SET_ERROR(NUMC_ERR_SHAPE, "bmm: shape mismatch (out.shape[-2:]=(%zu, %zu) expected (%zu, %zu))",
+                          out->shape[nd - 2], out->shape[nd - 1], a->shape[a->dim - 2], b->shape[b->dim - 1]);
+  return 0;
+}
+
+extern "C" NUMC_API int numc_bmm(const NumcArray *a, const NumcArray *b, NumcArray *out) {
+  int err = check_bmm(a, b, out);
+  if (err) return err;
+  const size_t nd = out->dim, nb = nd - 2;
+  const size_t M = a->shape[a->dim - 2], K = a->shape[a->dim - 1], N = b->shape[b->dim - 1];
+  size_t count = 1;
+  for (size_t i = 0; i < nb; i++) count *= out->shape[i];
+  if (count == 0 || M == 0 || N == 0) return 0;
+  std::vector<NumcArray> views(3 * count);
+  std::vector<NumcArray *> pa(count), pb(count), po(count);
+  for (size_t t = 0; t < count; t++) {
+    size_t r = t, aoff = 0, boff = 0, ooff = 0;
+    for (size_t i = nb; i-- > 0;) { // out batch axis i; a's is i + a.dim - nd (if any), likewise b's
+      const size_t idx = r % out->shape[i];
+      r /= out->shape[i];
+      ooff += idx * out->strides[i];
+      if (i + a->dim >= nd) {
+        const size_t ai = i + a->dim - nd;
+        if (a->shape[ai] != 1) aoff += idx * a->strides[ai];
+      }
+      if (i + b->dim >= nd) {
+        const size_t bi = i + b->dim - nd;
+        if (b->shape[bi] != 1) boff += idx * b->strides[bi];
+      }
+    }
+    view2(&views[3 * t], a, aoff, M, K, a->strides[a->dim - 2], a->strides[a->dim - 1]);
+    view2(&views[3 * t + 1], b, boff, K, N, b->strides[b->dim - 2], b->strides[b->dim - 1]);
+    view2(&views[3 * t + 2], out, ooff, M, N, out->strides[nd - 2], out->strides[nd - 1]);
+    pa[t] = &views[3 * t];
+    pb[t] = &views[3 * t + 1];
+    po[t] = &views[3 * t + 2];
+  }
+  if (count == 1) return numc_matmul_one(pa[0], pb[0], po[0], nullptr, true);
+  return numc_matmul_batched(out->ctx, count, pa.data(), pb.data(), po.data(), nullptr);
 }

@@ -183,3 +183,51 @@ def fp8_quantized_inputs(a, b):
     qa = table[enc(np.ldexp(a.astype(np.float32), ea[:, None]).astype(np.float32))]
     qb = table[enc(np.ldexp(b.astype(np.float32), eb[None, :]).astype(np.float32))]
     return np.ldexp(qa, -ea[:, None]), np.ldexp(qb, -eb[None, :])
+
+
+# ---- GEMM consumers of numc's roadmap (ROADMAP.md:139-140) --------------------
+# numc names numc_linear (out = x @ W + bias) and numc_bmm (batched matmul) but
+# implements neither; these restate the definitions on top of matmul_int / fref.
+# Pinned in tests by closed forms (x = 0 gives the broadcast bias, x = I gives
+# w + bias) and, for the batch broadcasting, by numpy.matmul on integers
+# (numpy accumulates in the array dtype, i.e. mod 2^w).
+
+def _wrap_add(x, y):
+    """x + y in x's integer dtype, mod 2^w"""
+    w = x.dtype.itemsize * 8
+    u = (x.astype(object) + np.asarray(y).astype(object)) % (1 << w)
+    return np.array(u.tolist(), dtype=np.uint64).astype(x.dtype.str.replace("i", "u")).view(x.dtype).reshape(x.shape)
+
+
+def linear(x, w, bias=None):
+    """(expected, |x||w| or None) of out = x @ w + bias for x (..., K), w (K, N)."""
+    K, N = w.shape
+    x2 = np.ascontiguousarray(x).reshape(-1, K)
+    out_shape = x.shape[:-1] + (N,)
+    if np.issubdtype(x.dtype, np.integer):
+        r = matmul_int(x2, np.ascontiguousarray(w))
+        if bias is not None:
+            r = _wrap_add(r, np.broadcast_to(bias, r.shape))
+        return r.reshape(out_shape), None
+    ref, absr = fref(x2, np.ascontiguousarray(w))
+    if bias is not None:
+        ref = ref + bias.astype(np.float64)
+    return ref.reshape(out_shape), absr.reshape(out_shape)
+
+
+def bmm(a, b):
+    """(expected, |a||b| or None) of numpy.matmul-style batched a @ b (both >= 2-D)."""
+    batch = np.broadcast_shapes(a.shape[:-2], b.shape[:-2])
+    M, K, N = a.shape[-2], a.shape[-1], b.shape[-1]
+    ab = np.broadcast_to(a, batch + (M, K))
+    bb = np.broadcast_to(b, batch + (K, N))
+    integer = np.issubdtype(a.dtype, np.integer)
+    out = np.zeros(batch + (M, N), a.dtype if integer else np.float64)
+    den = None if integer else np.zeros(batch + (M, N))
+    for idx in np.ndindex(*batch):
+        x, y = np.ascontiguousarray(ab[idx]), np.ascontiguousarray(bb[idx])
+        if integer:
+            out[idx] = matmul_int(x, y)
+        else:
+            out[idx], den[idx] = fref(x, y)
+    return out, den

@@ -280,6 +280,40 @@ def test_strided_views(ctx):
         assert np.all(full[mask] == 7), "bytes between output columns must be left alone"
 
 
+PACK_DTS = [np.int8, np.uint8, np.int16, np.uint32, np.float32]
+
+
+@pytest.mark.parametrize("dt", PACK_DTS, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("offset", [0, 1], ids=["aligned", "offset"])
+def test_pack_interior_and_edge_tiles(ctx, dt, offset):
+    """Operands large enough for whole interior pack tiles (16-byte aligned
+    rows take the vector-load path) beside ragged edge tiles; offset 1 starts
+    both operands one element into their buffers, so no row is 16-byte aligned
+    and every tile takes the element-wise path."""
+    rng = np.random.default_rng(170 + offset)
+    M, K, N = 96, 1040, 320
+    if np.issubdtype(dt, np.integer):
+        info = np.iinfo(dt)
+        base_a = rng.integers(info.min, info.max, (M, K + offset), dtype=dt, endpoint=True)
+        base_b = rng.integers(info.min, info.max, (K, N + offset), dtype=dt, endpoint=True)
+    else:
+        base_a = rng.standard_normal((M, K + offset)).astype(dt)
+        base_b = rng.standard_normal((K, N + offset)).astype(dt)
+    A = ctx.from_numpy(base_a).slice(1, offset, K + offset)
+    B = ctx.from_numpy(base_b).slice(1, offset, N + offset)
+    a, b = np.ascontiguousarray(base_a[:, offset:]), np.ascontiguousarray(base_b[:, offset:])
+    methods = [nc.METHOD_AUTO] if np.issubdtype(dt, np.integer) else [nc.METHOD_PLAIN, nc.METHOD_SPLIT_BF16]
+    for method in methods:
+        O = ctx.array((M, N), nc.DTYPE_OF_NP[np.dtype(dt)], zeros=True)
+        nc.check(nc.matmul(A, B, O, method), "matmul pack tiles")
+        got = O.numpy()
+        if np.issubdtype(dt, np.integer):
+            assert np.array_equal(got, oracle.matmul_int(a, b)), (dt, offset)
+        else:
+            bound = (K + 2) * U + (3 * 2.0 ** -16 if method == nc.METHOD_SPLIT_BF16 else 0.0)
+            assert cw_err(got, a, b) <= bound, (method, offset)
+
+
 def test_pointer_entry_host_and_device(ctx):
     """numc_amd_gemm (pointer-level entry for a host library keeping its own
     arrays) on host pointers with strided views, widening, and device pointers."""
