@@ -201,7 +201,53 @@ def bench_suite(nc, steps, warmup, seed):
                           "kernel_t_per_s": round(flops / t["kernel_ms_per_step"] * 1e-9, 2)
                           if t["kernel_ms_per_step"] else None,
                           "peak": peak, "config": t["config"]}), flush=True)
+    bench_consumers(nc, ctx, steps, warmup, seed)
     ctx.close()
+
+
+def time_calls(nc, ctx, fn, steps, warmup):
+    """ms per call of fn() on the context stream (HIP events), after warm-up"""
+    for _ in range(warmup):
+        nc.check(fn(), "warmup")
+    ctx.synchronize()
+    ctx.mark(0)
+    for _ in range(steps):
+        nc.check(fn(), "step")
+    ctx.mark(1)
+    ctx.synchronize()
+    return ctx.elapsed_ms(0, 1) / steps
+
+
+def bench_consumers(nc, ctx, steps, warmup, seed):
+    """numc_linear (a dense layer, x 8192x4096 @ w 4096x4096 + bias, float32)
+    and numc_bmm (attention scores, (8,16,1024,64) @ (8,16,64,1024), float32):
+    one step = one call, operands resident in HBM."""
+    M, K, N = 8192, 4096, 4096
+    x, w, bias, y = ctx.array((M, K), nc.FLOAT32), ctx.array((K, N), nc.FLOAT32), ctx.array((N,), nc.FLOAT32), \
+        ctx.array((M, N), nc.FLOAT32, zeros=True)
+    for i, a in enumerate((x, w, bias)):
+        a.fill_random(nc.DIST_NORMAL, seed, 10 + i, 0, 0)
+    for a in (x, w, bias, y):
+        a.to_device()
+    ms = time_calls(nc, ctx, lambda: nc.linear(x, w, bias, y), steps, warmup)
+    flops = 2.0 * M * N * K
+    print(json.dumps({"suite": "linear_f32", "shape": [M, K, N], "ms_per_step": round(ms, 5),
+                      "t_per_s": round(flops / ms * 1e-9, 2), "peak": 157.3,
+                      "frac": round(flops / ms * 1e-9 / 157.3, 4)}), flush=True)
+    B, H, S, D = 8, 16, 1024, 64
+    q, kt, s = ctx.array((B, H, S, D), nc.FLOAT32), ctx.array((B, H, D, S), nc.FLOAT32), \
+        ctx.array((B, H, S, S), nc.FLOAT32, zeros=True)
+    q.fill_random(nc.DIST_NORMAL, seed, 20, 0, 0)
+    kt.fill_random(nc.DIST_NORMAL, seed, 21, 0, 0)
+    for a in (q, kt, s):
+        a.to_device()
+    ms = time_calls(nc, ctx, lambda: nc.bmm(q, kt, s), steps, warmup)
+    flops = 2.0 * B * H * S * S * D
+    out_bytes = 4.0 * B * H * S * S
+    print(json.dumps({"suite": "bmm_f32", "shape": [B, H, S, D], "ms_per_step": round(ms, 5),
+                      "t_per_s": round(flops / ms * 1e-9, 2), "peak": 157.3,
+                      "out_GBps": round(out_bytes / ms * 1e-6, 1),
+                      "note": "K = 64: the 512 MiB score output dominates (HBM-bound)"}), flush=True)
 
 
 def read_traffic():

@@ -2,8 +2,9 @@
 // in one persistent launch, replayed from a cached HIP graph.
 //
 // No numc counterpart: numc's ROADMAP.md lists numc_bmm as the batched GEMM
-// client; numc_dot's batched loop (src/reduction/dot.c:409-421) calls the
-// single-problem kernel per batch.  Here the host builds, per call:
+// client (numc_bmm in matmul.cpp runs on this); numc_dot's batched loop
+// (src/reduction/dot.c:409-421) calls the single-problem kernel per batch and
+// runs on this too.  Here the host builds, per call:
 //   * a pack descriptor per operand (all packed by one launch),
 //   * a problem table (packed operand addresses, output view),
 //   * a work list of (problem, tile row, tile col, tile class) items ordered
