@@ -142,7 +142,9 @@ typedef enum {
   NUMC_FAMILY_FP8 = 2,
   NUMC_FAMILY_F64 = 3,
   NUMC_FAMILY_I8 = 4, /* every integer dtype up to 32 bits (int8 digit planes) */
-  NUMC_FAMILY_COUNT = 5
+  NUMC_FAMILY_BF16 = 5, /* bfloat16 arrays: one bf16 MFMA product, FP32 accumulate */
+  NUMC_FAMILY_F16 = 6,  /* float16 arrays: one fp16 MFMA product, FP32 accumulate */
+  NUMC_FAMILY_COUNT = 7
 } NumcFamily;
 
 typedef struct {
@@ -159,7 +161,9 @@ NUMC_API int numc_kernel_config_get(int family, int index, NumcKernelConfig *out
  * transposed, 4 padded rows), whole calls, mean of `reps`, into ms_out.
  * The selector predicts log2(ms) per configuration with an MLP cost model and
  * numc_matmul runs the argmin; without a trained model (or when it was
- * trained for other configuration tables) an analytic model is used. */
+ * trained for other configuration tables) an analytic model is used.  The
+ * model covers families 0-4; the bfloat16/float16 families use exact-shape
+ * pins, else the analytic model. */
 NUMC_API int numc_autotune_shape(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps,
                                  float *ms_out, int ms_len);
 NUMC_API int numc_autotune_release(NumcCtx *ctx);

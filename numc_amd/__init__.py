@@ -13,11 +13,27 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnumc_amd.so")
 
-# NumcDType (include/numc/dtype.h)
-INT8, INT16, INT32, INT64, UINT8, UINT16, UINT32, UINT64, FLOAT32, FLOAT64 = range(10)
+# NumcDType (include/numc/dtype.h); numpy has no bfloat16, so BFLOAT16 arrays
+# come and go as their uint16 bit patterns (bf16_bits / bf16_value convert)
+INT8, INT16, INT32, INT64, UINT8, UINT16, UINT32, UINT64, FLOAT32, FLOAT64, BFLOAT16, FLOAT16 = range(12)
 NP_DTYPES = [np.int8, np.int16, np.int32, np.int64, np.uint8, np.uint16, np.uint32,
-             np.uint64, np.float32, np.float64]
-DTYPE_OF_NP = {np.dtype(t): i for i, t in enumerate(NP_DTYPES)}
+             np.uint64, np.float32, np.float64, np.uint16, np.float16]
+DTYPE_OF_NP = {np.dtype(t): i for i, t in enumerate(NP_DTYPES[:10])}
+DTYPE_OF_NP[np.dtype(np.float16)] = FLOAT16
+
+
+def bf16_bits(x):
+    """float32 values -> bfloat16 bit patterns (uint16), round to nearest even"""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7F800000) == 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    q = ((u >> 16) | np.where((u & 0x7FFFFF) != 0, 0x40, 0)).astype(np.uint16)
+    return np.where(nan, q, r).astype(np.uint16)
+
+
+def bf16_value(b):
+    """bfloat16 bit patterns (uint16) -> float32 values"""
+    return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 # NumcErrorCode
 ERR_NONE, ERR_NULL, ERR_MALLOC, ERR_SHAPE, ERR_TYPE, ERR_BOUNDS, ERR_VALUE = 0, -1, -2, -3, -4, -5, -6
@@ -25,8 +41,9 @@ ERR_INTERNAL = -99
 
 # NumcMethod / NumcFamily / flags (include/numc_amd.h)
 METHOD_AUTO, METHOD_PLAIN, METHOD_SPLIT_BF16, METHOD_FP8_SCALED = 0, 1, 2, 3
-FAMILY_F32, FAMILY_SPLIT_BF16, FAMILY_FP8, FAMILY_F64, FAMILY_I8 = range(5)
-FAMILY_NAMES = ["f32", "split_bf16", "fp8", "f64", "i8"]
+FAMILY_F32, FAMILY_SPLIT_BF16, FAMILY_FP8, FAMILY_F64, FAMILY_I8, FAMILY_BF16, FAMILY_F16 = range(7)
+FAMILY_COUNT = 7
+FAMILY_NAMES = ["f32", "split_bf16", "fp8", "f64", "i8", "bf16", "f16"]
 OPT_WIDEN = 1
 OPT_DEVICE_POINTERS = 2
 DIST_UNIFORM, DIST_NORMAL, DIST_INT = 0, 1, 2
@@ -196,11 +213,16 @@ class Ctx:
             raise NumcFailure(-2, "numc_array_create")
         return Array(h, self)
 
-    def from_numpy(self, x):
+    def from_numpy(self, x, dtype=None):
+        """array with x's shape and elements; dtype overrides the numpy mapping
+        (BFLOAT16 takes uint16 bit patterns)"""
         x = np.asarray(x)
         if not x.flags.c_contiguous:   # (ascontiguousarray would turn 0-D into 1-D)
             x = x.copy(order="C")
-        a = self.array(x.shape, DTYPE_OF_NP[x.dtype])
+        dt = DTYPE_OF_NP[x.dtype] if dtype is None else dtype
+        if np.dtype(NP_DTYPES[dt]).itemsize != x.dtype.itemsize:
+            raise ValueError("from_numpy: %s elements for dtype %d" % (x.dtype, dt))
+        a = self.array(x.shape, dt)
         lib().numc_array_write(a.h, x.ctypes.data_as(C.c_void_p))
         return a
 
@@ -319,10 +341,11 @@ def bmm(a, b, out):
     return lib().numc_bmm(a.h, b.h, out.h)
 
 
-def gemm_pointers(ctx, a, b, c, method=METHOD_AUTO, config=-1, flags=0):
+def gemm_pointers(ctx, a, b, c, method=METHOD_AUTO, config=-1, flags=0, dtype=None):
     """numc_amd_gemm on numpy arrays (host pointers; byte strides taken from
-    the arrays, so strided views of a, b and c work as they are)."""
-    dt, odt = DTYPE_OF_NP[a.dtype], DTYPE_OF_NP[c.dtype]
+    the arrays, so strided views of a, b and c work as they are).  dtype
+    overrides the numpy mapping for all three (BFLOAT16: uint16 bit patterns)."""
+    dt, odt = (DTYPE_OF_NP[a.dtype], DTYPE_OF_NP[c.dtype]) if dtype is None else (dtype, dtype)
     M, K = a.shape
     N = b.shape[1]
     opts = MatmulOpts(method, config, flags)

@@ -25,6 +25,12 @@ __global__ void k_fill_pattern(unsigned char *p, long long n, int dtype) {
   switch (dtype) {
   case NUMC_DTYPE_FLOAT32: ((float *)p)[i] = (float)((h >> 40) * 0x1p-23) - 1.0f; break;
   case NUMC_DTYPE_FLOAT64: ((double *)p)[i] = (double)(h >> 11) * 0x1p-52 - 1.0; break;
+  case NUMC_DTYPE_BFLOAT16: // sign, exponent 126 or 127 (magnitudes in [0.5, 2)), random mantissa
+    ((uint16_t *)p)[i] = (uint16_t)(((h >> 8) & 0x8000u) | (0x3F00u + ((h >> 16) & 0xFFu)));
+    break;
+  case NUMC_DTYPE_FLOAT16: // sign, exponent 14 or 15 (magnitudes in [0.5, 2)), random mantissa
+    ((uint16_t *)p)[i] = (uint16_t)(((h >> 8) & 0x8000u) | (0x3800u + ((h >> 16) & 0x7FFu)));
+    break;
   default: ((int8_t *)p)[i] = (int8_t)(h & 0xFF); break;
   }
 }
@@ -85,8 +91,11 @@ extern "C" NUMC_API int numc_autotune_shape(NumcCtx *ctx, int family, int M, int
     HIP_TRY(hipEventCreate(&tb->e0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&tb->e1), "hipEventCreate");
   }
-  const NumcDType dt = family == NUMC_FAMILY_F64 ? NUMC_DTYPE_FLOAT64
-                       : family == NUMC_FAMILY_I8 ? NUMC_DTYPE_INT8 : NUMC_DTYPE_FLOAT32;
+  const NumcDType dt = family == NUMC_FAMILY_F64    ? NUMC_DTYPE_FLOAT64
+                       : family == NUMC_FAMILY_I8   ? NUMC_DTYPE_INT8
+                       : family == NUMC_FAMILY_BF16 ? NUMC_DTYPE_BFLOAT16
+                       : family == NUMC_FAMILY_F16  ? NUMC_DTYPE_FLOAT16
+                                                    : NUMC_DTYPE_FLOAT32;
   const size_t es = numc_dtype_size(dt);
   const long long pad = (flags & 4) ? 16 : 0;
   GemmProblem p{};

@@ -20,13 +20,6 @@ static inline unsigned blocks_for(long long n) { return (unsigned)cdiv(n, 256); 
 // number formats (device)
 // ===========================================================================
 
-__device__ inline uint32_t bf16_rne_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (u >> 16) | ((u & 0x7FFFFFu) ? 0x40u : 0u);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return u >> 16;
-}
-
 // binary32 -> OCP e4m3fn, round to nearest even, saturating to +-448
 __device__ inline uint32_t e4m3_rne(float x) {
   uint32_t u = __float_as_uint(x);
@@ -116,9 +109,9 @@ __device__ inline void pack_unit_split(const PackArgs &a, long long t) {
       if (k0 + e < a.K) x0 = ld<float>(a.src, row * a.rs + (long long)(k0 + e) * a.ks);
       if (k0 + e + 1 < a.K) x1 = ld<float>(a.src, row * a.rs + (long long)(k0 + e + 1) * a.ks);
     }
-    uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
-    uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
-    uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
+    uint32_t h0 = f32_to_bf16_bits(x0), h1 = f32_to_bf16_bits(x1);
+    uint32_t l0 = f32_to_bf16_bits(x0 - __uint_as_float(h0 << 16));
+    uint32_t l1 = f32_to_bf16_bits(x1 - __uint_as_float(h1 << 16));
     hi[e / 2] = (h0 & 0xFFFFu) | (h1 << 16);
     lo[e / 2] = (l0 & 0xFFFFu) | (l1 << 16);
   }
@@ -249,9 +242,9 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
 #pragma unroll
       for (int e = 0; e < 8; e += 2) {
         float x0 = t[e], x1 = t[e + 1];
-        uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
-        uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
-        uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
+        uint32_t h0 = f32_to_bf16_bits(x0), h1 = f32_to_bf16_bits(x1);
+        uint32_t l0 = f32_to_bf16_bits(x0 - __uint_as_float(h0 << 16));
+        uint32_t l1 = f32_to_bf16_bits(x1 - __uint_as_float(h1 << 16));
         hi[e / 2] = (h0 & 0xFFFFu) | (h1 << 16);
         lo[e / 2] = (l0 & 0xFFFFu) | (l1 << 16);
       }
@@ -349,9 +342,9 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
 #pragma unroll
       for (int e = 0; e < 8; e += 2) {
         float x0 = t[e], x1 = t[e + 1];
-        uint32_t h0 = bf16_rne_bits(x0), h1 = bf16_rne_bits(x1);
-        uint32_t l0 = bf16_rne_bits(x0 - __uint_as_float(h0 << 16));
-        uint32_t l1 = bf16_rne_bits(x1 - __uint_as_float(h1 << 16));
+        uint32_t h0 = f32_to_bf16_bits(x0), h1 = f32_to_bf16_bits(x1);
+        uint32_t l0 = f32_to_bf16_bits(x0 - __uint_as_float(h0 << 16));
+        uint32_t l1 = f32_to_bf16_bits(x1 - __uint_as_float(h1 << 16));
         hi[e / 2] = (h0 & 0xFFFFu) | (h1 << 16);
         lo[e / 2] = (l0 & 0xFFFFu) | (l1 << 16);
       }
@@ -861,6 +854,21 @@ __global__ void k_naive(const T *a, const T *b, T *c, int M, int K, int N) {
   }
 }
 
+// bfloat16 / float16 (no numc counterpart): binary32 fused multiply-adds of
+// the widened values in k order, one rounding to nearest even at the store.
+template <bool BF>
+__global__ void k_naive_h16(const uint16_t *a, const uint16_t *b, uint16_t *c, int M, int K, int N) {
+  long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t >= (long long)M * N) return;
+  long long i = t / N, j = t % N;
+  float acc = 0.f;
+  for (int k = 0; k < K; k++) {
+    const uint32_t x = a[i * K + k], y = b[(long long)k * N + j];
+    acc = fmaf(BF ? bf16_bits_to_f32(x) : f16_bits_to_f32(x), BF ? bf16_bits_to_f32(y) : f16_bits_to_f32(y), acc);
+  }
+  c[t] = (uint16_t)(BF ? f32_to_bf16_bits(acc) : f32_to_f16_bits(acc));
+}
+
 // 64-bit integers: no MFMA; 16x16 output tile per block, k staged in LDS.
 template <class T>
 __global__ void k_gemm_i64(const T *a, const T *b, T *c, int M, int K, int N, long long rsa, long long csa,
@@ -1011,6 +1019,30 @@ static const GemmConfig kI8[] = {
     CFGB(MfI8, 256, 256, 64, 4, 2, EPI_INT, 1, 4),
     CFGB(MfI8, 256, 256, 128, 4, 2, EPI_INT, 1, 2),
 };
+
+// bfloat16 / float16 arrays: one MFMA product (fragments twice as dense in k
+// as the split planes, so the k-chunks are twice as long)
+static const GemmConfig kBF16[] = {
+    CFGB(MfBF16, 256, 256, 64, 2, 4, EPI_BF16, 1, 2),
+    CFGB(MfBF16, 256, 256, 32, 2, 4, EPI_BF16, 1, 4),
+    CFGB(MfBF16, 256, 128, 64, 4, 2, EPI_BF16, 1, 2),
+    CFGB(MfBF16, 128, 256, 64, 2, 4, EPI_BF16, 1, 2),
+    CFGB(MfBF16, 128, 128, 64, 2, 2, EPI_BF16, 1, 3),
+    CFGB(MfBF16, 64, 64, 64, 2, 2, EPI_BF16, 1, 4),
+    CFGB(MfBF16, 128, 128, 64, 2, 2, EPI_BF16, 4, 3),
+    CFGB(MfBF16, 64, 64, 64, 2, 2, EPI_BF16, 8, 4),
+};
+static const GemmConfig kF16[] = {
+    CFGB(MfF16, 256, 256, 64, 2, 4, EPI_F16, 1, 2),
+    CFGB(MfF16, 256, 256, 32, 2, 4, EPI_F16, 1, 4),
+    CFGB(MfF16, 256, 128, 64, 4, 2, EPI_F16, 1, 2),
+    CFGB(MfF16, 128, 256, 64, 2, 4, EPI_F16, 1, 2),
+    CFGB(MfF16, 128, 128, 64, 2, 2, EPI_F16, 1, 3),
+    CFGB(MfF16, 64, 64, 64, 2, 2, EPI_F16, 1, 4),
+    CFGB(MfF16, 128, 128, 64, 2, 2, EPI_F16, 4, 3),
+    CFGB(MfF16, 64, 64, 64, 2, 2, EPI_F16, 8, 4),
+};
+
 #undef CFGB
 
 struct FamilyTable {
@@ -1023,6 +1055,8 @@ static const FamilyTable kFamilies[NUMC_FAMILY_COUNT] = {
     {kFP8, (int)(sizeof kFP8 / sizeof kFP8[0])},
     {kF64, (int)(sizeof kF64 / sizeof kF64[0])},
     {kI8, (int)(sizeof kI8 / sizeof kI8[0])},
+    {kBF16, (int)(sizeof kBF16 / sizeof kBF16[0])},
+    {kF16, (int)(sizeof kF16 / sizeof kF16[0])},
 };
 
 const GemmConfig *numc_gemm_config(int family, int idx) {
@@ -1066,6 +1100,8 @@ static int family_kf(int family) {
   case NUMC_FAMILY_SPLIT_BF16: return MfBF16::KF;
   case NUMC_FAMILY_FP8: return MfFP8::KF;
   case NUMC_FAMILY_F64: return MfF64::KF;
+  case NUMC_FAMILY_BF16: return MfBF16::KF;
+  case NUMC_FAMILY_F16: return MfF16::KF;
   default: return MfI8::KF;
   }
 }
@@ -1386,6 +1422,106 @@ __global__ void __launch_bounds__(256) k_pack_bytes_rows(PackArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// bfloat16 / float16 operands: raw 16-bit copies.  A fragment lane is 8
+// values (16 bytes) of one row, so the byte geometry is that of the int8
+// fragments: 32-byte row segments, lane halves of 16 bytes.
+// ---------------------------------------------------------------------------
+
+// k-contiguous rows: the int8 byte tile with K and the row stride in bytes
+__global__ void __launch_bounds__(256) k_pack_raw16_k(PackArgs a) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  a.K *= 2;
+  a.rs *= 2;
+  bytes_tile_k(a, blockIdx.x, tile);
+}
+
+__device__ __attribute__((always_inline)) inline uint32_t u4_at(const uint4 &v, int i) {
+  return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+
+// row-contiguous rows (B^T of a row-major B): a 128-row x 64-k tile (4 k
+// fragments).  Each thread reads 8 consecutive rows at 4 consecutive k (four
+// 16-byte loads), regroups them into 4 k per row and writes 8 bytes per row
+// into LDS; each fragment lane then reads its 8 k of one row.  The 16-byte
+// column chunks of row r sit at chunk ^ ((r >> 3) & 7).
+__device__ __attribute__((always_inline)) inline void raw16_tile_rows(const PackArgs &a, long long bid, uint32_t *tile) {
+  constexpr int TR = 128, TK = 64, PITCH = TK * 2 + 16; // bytes per LDS row
+  static_assert(TR * PITCH / 4 <= PACK_TILE_WORDS, "tile");
+  unsigned char *t8 = (unsigned char *)tile;
+  const int nrb = (a.nRT + 3) / 4;
+  const long long rb = bid % nrb;
+  const int kt = (int)(bid / nrb);
+  const long long row0 = rb * TR;
+  const int k0 = kt * TK;
+  const int tid = threadIdx.x;
+  const int rg = tid & 15, kq = tid >> 4; // 16 groups of 8 rows x 16 groups of 4 k
+  const uint16_t *src = (const uint16_t *)a.src;
+  const uint16_t *base = src + (long long)k0 * a.ks + row0;
+  uint4 v[4];
+  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.ks * 2)) & 15) == 0) {
+    // interior tile, 16-byte aligned: all four loads in flight at once
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = *(const uint4 *)(base + (long long)(kq * 4 + q) * a.ks + rg * 8);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = k0 + kq * 4 + q;
+      const long long r = row0 + rg * 8;
+      v[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (k < a.K && r < a.R) v[q] = load_chunk_edge<uint16_t>(src + (long long)k * a.ks + r, a.R - r);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) { // row rg*8 + j: the j-th 16-bit value of each load
+    uint32_t h[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) h[q] = (u4_at(v[q], j >> 1) >> (16 * (j & 1))) & 0xFFFFu;
+    const int row = rg * 8 + j;
+    const int chunk = (kq >> 1) ^ (rg & 7);
+    *(uint2 *)(t8 + row * PITCH + (chunk << 4) + (kq & 1) * 8) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+  }
+  __syncthreads();
+  // 4 row fragments x 4 k fragments x 64 lanes: four lane slots per thread
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    const int u = tid + it * 256;
+    const int f = u >> 6, lane = u & 63;
+    const int rl = f >> 2, fk = f & 3;
+    const long long rt = rb * 4 + rl;
+    const int kf = kt * 4 + fk;
+    if (rt < a.nRT && kf < a.nKF) {
+      const int row = rl * 32 + (lane & 31);
+      const int chunk = (fk * 2 + (lane >> 5)) ^ ((row >> 3) & 7);
+      *(uint4 *)(a.dst + (rt * a.nKF + kf) * 1024 + lane * 16) = *(const uint4 *)(t8 + row * PITCH + (chunk << 4));
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pack_raw16_rows(PackArgs a) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  raw16_tile_rows(a, blockIdx.x, tile);
+}
+
+// any other strides: one thread per fragment lane slot
+__global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
+  const long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t >= (long long)a.nRT * a.nKF * 64) return;
+  const int lane = (int)(t & 63);
+  const long long fr = t >> 6;
+  const int kf = (int)(fr % a.nKF);
+  const long long row = (fr / a.nKF) * 32 + (lane & 31);
+  const int k0 = kf * 16 + (lane >> 5) * 8;
+  const uint16_t *src = (const uint16_t *)a.src;
+  uint32_t w[4] = {0, 0, 0, 0};
+  if (row < a.R) {
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+      if (k0 + e < a.K) w[e >> 1] |= (uint32_t)src[row * a.rs + (long long)(k0 + e) * a.ks] << (16 * (e & 1));
+  }
+  *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ---------------------------------------------------------------------------
 // batched pack: every operand of every problem in one launch.  Operand i owns
 // blocks [prefix[i], prefix[i+1]); its `mode` (set on the host) says which
 // tile walk it takes: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 one
@@ -1505,6 +1641,15 @@ static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, h
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_f32_tiled_rows<0>), grid, dim3(256), 0, st, pk);
     else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_f32_tiled_rows<1>), grid, dim3(256), 0, st, pk);
     else hipLaunchKernelGGL((k_pack_f32_tiled_rows<2>), grid, dim3(256), 0, st, pk);
+    return;
+  }
+  if (family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16) { // raw 16-bit copies
+    if (pk.ks == 1)
+      hipLaunchKernelGGL(k_pack_raw16_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16))), dim3(256), 0, st, pk);
+    else if (pk.rs == 1)
+      hipLaunchKernelGGL(k_pack_raw16_rows, dim3((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4))), dim3(256), 0, st, pk);
+    else
+      hipLaunchKernelGGL(k_pack_raw16, dim3(nb), dim3(256), 0, st, pk);
     return;
   }
   switch (family) {
@@ -1658,6 +1803,14 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
         else hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64>), dim3(nb), dim3(256), 0, st, g);
         break;
       case NUMC_FAMILY_FP8: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_FP8>), dim3(nb), dim3(256), 0, st, g); break;
+      case NUMC_FAMILY_BF16:
+        if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16B>), dim3(nb), dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16>), dim3(nb), dim3(256), 0, st, g);
+        break;
+      case NUMC_FAMILY_F16:
+        if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F16B>), dim3(nb), dim3(256), 0, st, g);
+        else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F16>), dim3(nb), dim3(256), 0, st, g);
+        break;
       case NUMC_FAMILY_I8:
         if (bias) hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INTB>), dim3(nb), dim3(256), 0, st, g);
         else hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INT>), dim3(nb), dim3(256), 0, st, g);
@@ -1690,14 +1843,24 @@ int numc_naive_launch(NumcCtx *ctx, NumcDType dt, const void *a, const void *b, 
   case NUMC_DTYPE_UINT64: hipLaunchKernelGGL((k_naive<uint64_t, false>), grid, blk, 0, s, (const uint64_t *)a, (const uint64_t *)b, (uint64_t *)c, M, K, N); break;
   case NUMC_DTYPE_FLOAT32: hipLaunchKernelGGL((k_naive<float, true>), grid, blk, 0, s, (const float *)a, (const float *)b, (float *)c, M, K, N); break;
   case NUMC_DTYPE_FLOAT64: hipLaunchKernelGGL((k_naive<double, true>), grid, blk, 0, s, (const double *)a, (const double *)b, (double *)c, M, K, N); break;
+  case NUMC_DTYPE_BFLOAT16: hipLaunchKernelGGL((k_naive_h16<true>), grid, blk, 0, s, (const uint16_t *)a, (const uint16_t *)b, (uint16_t *)c, M, K, N); break;
+  case NUMC_DTYPE_FLOAT16: hipLaunchKernelGGL((k_naive_h16<false>), grid, blk, 0, s, (const uint16_t *)a, (const uint16_t *)b, (uint16_t *)c, M, K, N); break;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "naive launch");
   return 0;
 }
 
+// 16-bit float elements for k_scale (bits; the scalar comes as binary32)
+struct Bf16Elem { uint16_t b; };
+struct F16Elem { uint16_t b; };
+
 template <class T> __device__ inline T scale_one(T x, uint64_t bits) {
-  if constexpr (std::is_integral_v<T> && sizeof(T) == 8) {
+  if constexpr (std::is_same_v<T, Bf16Elem>) {
+    return Bf16Elem{(uint16_t)f32_to_bf16_bits(bf16_bits_to_f32(x.b) * __uint_as_float((uint32_t)bits))};
+  } else if constexpr (std::is_same_v<T, F16Elem>) {
+    return F16Elem{(uint16_t)f32_to_f16_bits(f16_bits_to_f32(x.b) * __uint_as_float((uint32_t)bits))};
+  } else if constexpr (std::is_integral_v<T> && sizeof(T) == 8) {
     return (T)((uint64_t)x * bits); // low 64 bits
   } else if constexpr (std::is_integral_v<T>) {
     return (T)(uint32_t)((uint32_t)(int32_t)x * (uint32_t)bits); // low w bits
@@ -1736,6 +1899,8 @@ int numc_scale_launch(NumcCtx *ctx, const ScaleArgs &a) {
   case NUMC_DTYPE_UINT32: hipLaunchKernelGGL(k_scale<uint32_t>, dim3(nb), dim3(256), 0, s, a); break;
   case NUMC_DTYPE_UINT64: hipLaunchKernelGGL(k_scale<uint64_t>, dim3(nb), dim3(256), 0, s, a); break;
   case NUMC_DTYPE_FLOAT32: hipLaunchKernelGGL(k_scale<float>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_BFLOAT16: hipLaunchKernelGGL(k_scale<Bf16Elem>, dim3(nb), dim3(256), 0, s, a); break;
+  case NUMC_DTYPE_FLOAT16: hipLaunchKernelGGL(k_scale<F16Elem>, dim3(nb), dim3(256), 0, s, a); break;
   default: hipLaunchKernelGGL(k_scale<double>, dim3(nb), dim3(256), 0, s, a); break;
   }
   hipError_t e = hipGetLastError();

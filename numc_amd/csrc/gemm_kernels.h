@@ -22,6 +22,7 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 #define NUMC_FRAG_BYTES 1024
@@ -53,6 +54,19 @@ struct MfBF16 { // v_mfma_f32_32x32x16_bf16
     bf16x8 av = __builtin_bit_cast(bf16x8, a[0]);
     bf16x8 bv = __builtin_bit_cast(bf16x8, b[0]);
     c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+  __device__ static inline int col(int lane) { return lane & 31; }
+};
+
+struct MfF16 { // v_mfma_f32_32x32x16_f16 (fragment layout as MfBF16)
+  static constexpr int T = 32, KF = 16, FR = 1, NACC = 16, NMMA = 1;
+  using Acc = f32x16;
+  using Raw = float;
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+    f16x8 av = __builtin_bit_cast(f16x8, a[0]);
+    f16x8 bv = __builtin_bit_cast(f16x8, b[0]);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, c, 0, 0, 0);
   }
   __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
   __device__ static inline int col(int lane) { return lane & 31; }
@@ -104,10 +118,27 @@ struct MfF64 { // v_mfma_f64_16x16x4_f64 x2 per fragment
 // ---------------------------------------------------------------------------
 
 enum GemmMode { MODE_SINGLE = 0, MODE_SPLIT3 = 1 };
-// EPI_F32B / EPI_F64B / EPI_INTB: the same outputs plus a per-column bias
-// (numc_linear), separate kernels so that the plain stores keep their code
-// and register budget
-enum GemmEpi { EPI_F32 = 0, EPI_F64 = 1, EPI_FP8 = 2, EPI_INT = 3, EPI_F32B = 4, EPI_F64B = 5, EPI_INTB = 7 };
+// EPI_BF16 / EPI_F16: FP32 sums rounded to nearest even into 16-bit floats.
+// EPI_*B (= EPI_* + 4): the same outputs plus a per-column bias (numc_linear),
+// separate kernels so that the plain stores keep their code and register budget.
+enum GemmEpi {
+  EPI_F32 = 0, EPI_F64 = 1, EPI_FP8 = 2, EPI_INT = 3,
+  EPI_F32B = 4, EPI_F64B = 5, EPI_INTB = 7,
+  EPI_BF16 = 8, EPI_F16 = 9, EPI_BF16B = 12, EPI_F16B = 13
+};
+
+// binary32 -> bfloat16 / binary16 bits, round to nearest even (NaN stays NaN)
+__device__ inline uint32_t f32_to_bf16_bits(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (u >> 16) | ((u & 0x7FFFFFu) ? 0x40u : 0u);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ inline uint32_t f32_to_f16_bits(float x) {
+  return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)x);
+}
+__device__ inline float bf16_bits_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
+__device__ inline float f16_bits_to_f32(uint32_t b) { return (float)__builtin_bit_cast(_Float16, (unsigned short)b); }
 
 struct GemmArgs {
   const unsigned char *A, *B; // packed operands
@@ -177,12 +208,18 @@ __device__ inline int int_products(int s, int maskA, int maskB, int *pa0, int *p
 template <int EPI> struct EpiBias { using T = uint32_t; };
 template <> struct EpiBias<EPI_F32B> { using T = float; };
 template <> struct EpiBias<EPI_F64B> { using T = double; };
+template <> struct EpiBias<EPI_BF16B> { using T = float; };
+template <> struct EpiBias<EPI_F16B> { using T = float; };
 
 template <int EPI>
 __device__ inline typename EpiBias<EPI>::T epi_bias(const GemmArgs &g, int col) {
   using T = typename EpiBias<EPI>::T;
   if constexpr (EPI == EPI_F32B || EPI == EPI_F64B) {
     return col < g.N ? ((const T *)g.bias)[(long long)col * g.bias_s] : (T)0;
+  } else if constexpr (EPI == EPI_BF16B || EPI == EPI_F16B) {
+    if (col >= g.N) return 0.f;
+    const uint32_t b = ((const uint16_t *)g.bias)[(long long)col * g.bias_s];
+    return EPI == EPI_BF16B ? bf16_bits_to_f32(b) : f16_bits_to_f32(b);
   } else if constexpr (EPI == EPI_INTB) {
     if (g.shift_pass || col >= g.N) return 0u;
     const long long b = (long long)col * g.bias_s;
@@ -207,6 +244,10 @@ __device__ inline void epi_store(const GemmArgs &g, int row, int col, Raw v, typ
     ((double *)g.C)[off] = (double)v + bias;
   } else if constexpr (EPI == EPI_FP8) {
     ((float *)g.C)[off] = ldexpf((float)v, -(g.row_exp[row] + g.col_exp[col]));
+  } else if constexpr (EPI == EPI_BF16 || EPI == EPI_BF16B || EPI == EPI_F16 || EPI == EPI_F16B) {
+    float r = (float)v;
+    if constexpr (EPI == EPI_BF16B || EPI == EPI_F16B) r += bias;
+    ((uint16_t *)g.C)[off] = (uint16_t)(EPI == EPI_BF16 || EPI == EPI_BF16B ? f32_to_bf16_bits(r) : f32_to_f16_bits(r));
   } else {
     // integer digit pass s adds (sum << 8s); the first pass writes the output,
     // with the bias (mod 2^w) under EPI_INTB

@@ -60,6 +60,8 @@ static int pick_family_dt(NumcDType dt, const NumcMatmulOpts *o) {
     if (method != NUMC_METHOD_AUTO && method != NUMC_METHOD_PLAIN)
       return NUMC_SET_ERROR(NUMC_ERR_TYPE, "matmul: method %d needs float32 operands", method);
     if (dt == NUMC_DTYPE_FLOAT64) return NUMC_FAMILY_F64;
+    if (dt == NUMC_DTYPE_BFLOAT16) return NUMC_FAMILY_BF16;
+    if (dt == NUMC_DTYPE_FLOAT16) return NUMC_FAMILY_F16;
     if (dt == NUMC_DTYPE_INT64 || dt == NUMC_DTYPE_UINT64) return 100;
     return NUMC_FAMILY_I8;
   }
@@ -280,7 +282,7 @@ extern "C" NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, si
   if (!ctx || (M && K && !a) || (K && N && !b) || (M && N && !c))
     return NUMC_SET_ERROR(NUMC_ERR_NULL, "amd_gemm: NULL pointer (ctx=%p a=%p b=%p c=%p)", (void *)ctx, a, b, c);
   const bool widen = opts && (opts->flags & NUMC_OPT_WIDEN);
-  bool types_ok = dtype >= 0 && dtype <= NUMC_DTYPE_FLOAT64 && (dtype == out_dtype);
+  bool types_ok = dtype >= 0 && dtype < NUMC_DTYPE_COUNT && (dtype == out_dtype);
   if (!types_ok && widen)
     types_ok = (dtype == NUMC_DTYPE_INT8 || dtype == NUMC_DTYPE_UINT8) &&
                (out_dtype == NUMC_DTYPE_INT32 || out_dtype == NUMC_DTYPE_UINT32);
@@ -424,6 +426,19 @@ static int dot_scalar_bits(const NumcArray *sc, uint64_t *bits) {
   case NUMC_DTYPE_UINT64: v = (double)*(uint64_t *)raw; break;
   case NUMC_DTYPE_FLOAT32: v = *(float *)raw; break;
   case NUMC_DTYPE_FLOAT64: v = *(double *)raw; break;
+  case NUMC_DTYPE_BFLOAT16: {
+    const uint32_t u = (uint32_t)*(uint16_t *)raw << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    v = f;
+    break;
+  }
+  case NUMC_DTYPE_FLOAT16: {
+    _Float16 h;
+    memcpy(&h, raw, 2);
+    v = (double)h;
+    break;
+  }
   }
   switch (sc->dtype) {
   case NUMC_DTYPE_INT64:
@@ -432,7 +447,9 @@ static int dot_scalar_bits(const NumcArray *sc, uint64_t *bits) {
   case NUMC_DTYPE_UINT64:
     *bits = v >= 18446744073709551616.0 ? 0 : (uint64_t)v;
     break;
-  case NUMC_DTYPE_FLOAT32: {
+  case NUMC_DTYPE_FLOAT32:
+  case NUMC_DTYPE_BFLOAT16:
+  case NUMC_DTYPE_FLOAT16: { // 16-bit floats scale in binary32 (exact for the scalar), one rounding
     float f = (float)v;
     uint32_t u;
     memcpy(&u, &f, 4);

@@ -826,6 +826,27 @@ struct Xoshiro {
 
 template <class T> void put(void *p, size_t i, T v) { ((T *)p)[i] = v; }
 
+// binary32 -> bfloat16 / binary16 bits, round to nearest even (NaN stays NaN)
+uint16_t f32_to_bf16(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0x7FFFFFu) ? 0x40u : 0u));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+uint16_t f32_to_f16(float x) {
+  _Float16 h = (_Float16)x; // the compiler's conversion rounds to nearest even
+  uint16_t b;
+  memcpy(&b, &h, 2);
+  return b;
+}
+// double -> binary32, kept as a separate rounding: the optimiser folds
+// (_Float16)(float)d into one double -> half rounding, which differs on ties
+float narrow(double v) {
+  volatile float f = (float)v;
+  return f;
+}
+
 void store_val(void *p, NumcDType dt, size_t i, double fv, int64_t iv, bool is_int) {
   switch (dt) {
   case NUMC_DTYPE_INT8: put<int8_t>(p, i, (int8_t)iv); break;
@@ -838,6 +859,9 @@ void store_val(void *p, NumcDType dt, size_t i, double fv, int64_t iv, bool is_i
   case NUMC_DTYPE_UINT64: put<uint64_t>(p, i, (uint64_t)iv); break;
   case NUMC_DTYPE_FLOAT32: put<float>(p, i, is_int ? (float)iv : (float)fv); break;
   case NUMC_DTYPE_FLOAT64: put<double>(p, i, is_int ? (double)iv : fv); break;
+  // 16-bit floats: through binary32, then rounded to nearest even
+  case NUMC_DTYPE_BFLOAT16: put<uint16_t>(p, i, f32_to_bf16(is_int ? (float)iv : (float)fv)); break;
+  case NUMC_DTYPE_FLOAT16: put<uint16_t>(p, i, f32_to_f16(narrow(is_int ? (double)iv : fv))); break;
   }
 }
 } // namespace
@@ -864,9 +888,11 @@ extern "C" NUMC_API int numc_array_fill_random(NumcArray *arr, int dist, uint64_
       for (size_t i = b << 16; i < end;) {
         if (dist == 0) {
           uint64_t x = r.next();
-          if (dt == NUMC_DTYPE_FLOAT32) {
+          if (dt == NUMC_DTYPE_FLOAT32 || dt == NUMC_DTYPE_BFLOAT16 || dt == NUMC_DTYPE_FLOAT16) {
             float u = (float)(x >> 40) * 0x1p-24f;
-            put<float>(p, i, (float)lo + ((float)hi - (float)lo) * u);
+            float v = (float)lo + ((float)hi - (float)lo) * u;
+            if (dt == NUMC_DTYPE_FLOAT32) put<float>(p, i, v);
+            else put<uint16_t>(p, i, dt == NUMC_DTYPE_BFLOAT16 ? f32_to_bf16(v) : f32_to_f16(v));
           } else {
             double u = (double)(x >> 11) * 0x1p-53;
             store_val(p, dt, i, lo + (hi - lo) * u, 0, false);

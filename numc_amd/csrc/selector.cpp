@@ -25,6 +25,9 @@
 
 namespace {
 constexpr int kDin = 18;
+// families the cost model covers (f32, split-bf16, fp8, f64, int8 digits);
+// the 16-bit float families use the exact-shape table, else the analytic model
+constexpr int kFam = 5;
 
 struct HostMlp {
   int din = 0, H = 0, dout = 0;
@@ -92,13 +95,13 @@ bool load_file(const char *path, HostMlp *m, char *why, size_t whylen) {
   char magic[4];
   int hdr[5];
   bool ok = fread(magic, 1, 4, f) == 4 && memcmp(magic, "NMLS", 4) == 0 && fread(hdr, sizeof(int), 5, f) == 5 &&
-            hdr[0] == 1 && hdr[1] == kDin && hdr[3] == 1 && hdr[4] == NUMC_FAMILY_COUNT;
+            hdr[0] == 1 && hdr[1] == kDin && hdr[3] == 1 && hdr[4] == kFam;
   if (!ok) {
     snprintf(why, whylen, "bad header");
     fclose(f);
     return false;
   }
-  for (int fam = 0; fam < NUMC_FAMILY_COUNT && ok; fam++) {
+  for (int fam = 0; fam < kFam && ok; fam++) {
     int n = 0;
     ok = fread(&n, sizeof(int), 1, f) == 1 && n == numc_gemm_config_count(fam);
     for (int i = 0; ok && i < n; i++) {
@@ -174,7 +177,7 @@ extern "C" NUMC_API int numc_selector_features(int family, int M, int N, int K, 
   out[0] = (float)(std::log2(m) / 16);
   out[1] = (float)(std::log2(n) / 16);
   out[2] = (float)(std::log2(k) / 16);
-  for (int f = 0; f < NUMC_FAMILY_COUNT; f++) out[3 + f] = f == family ? 1.f : 0.f;
+  for (int f = 0; f < kFam; f++) out[3 + f] = f == family ? 1.f : 0.f;
   out[8] = (flags & 1) ? 1.f : 0.f;
   out[9] = (flags & 2) ? 1.f : 0.f;
   out[10] = (flags & 4) ? 1.f : 0.f;
@@ -253,6 +256,7 @@ extern "C" NUMC_API int numc_selector_predict(int family, int M, int N, int K, i
   std::lock_guard<std::mutex> lk(g_mu);
   ensure_loaded();
   if (!g_loaded) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_predict: no trained selector loaded");
+  if (family < 0 || family >= kFam) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_predict: family %d not modelled", family);
   const int n = numc_gemm_config_count(family);
   if (len < n) return NUMC_SET_ERROR(NUMC_ERR_BOUNDS, "selector_predict: need %d slots", n);
   float x[kDin];
@@ -269,7 +273,7 @@ int numc_selector_choose(int family, int M, int N, int K, int flags) {
   const ShapeKey key{family, flags, M, N, K};
   auto tt = g_tuned.find(key);
   if (tt != g_tuned.end()) return tt->second;
-  if (!g_loaded) return -1;
+  if (!g_loaded || family >= kFam) return -1;
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return it->second;
   const int n = numc_gemm_config_count(family);
@@ -296,7 +300,7 @@ int numc_selector_choose(int family, int M, int N, int K, int flags) {
 // numc_matmul_batched_tune measures a better one on a real batch: the cost
 // model, trained on single calls, favours small tiles that keep one problem's
 // GPU busy, which is not what a full batch needs.
-static int g_batch_thresh[NUMC_FAMILY_COUNT] = {64, 64, 64, 64, 64};
+static int g_batch_thresh[NUMC_FAMILY_COUNT] = {64, 64, 64, 64, 64, 64, 64};
 
 int numc_batch_threshold(int family) {
   std::lock_guard<std::mutex> lk(g_mu);

@@ -170,7 +170,7 @@ def test_generator_matches_oracle(ctx, dt, dist, lo, hi):
 
 
 def test_config_tables():
-    for fam in range(5):
+    for fam in range(nc.FAMILY_COUNT):
         n = nc.config_count(fam)
         assert n >= 3
         for i in range(n):
@@ -193,7 +193,7 @@ def test_pointer_entry_error_contract(ctx):
     assert call(None, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, pa, pb, pc) == nc.ERR_NULL
     assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, None, pb, pc) == nc.ERR_NULL
     assert call(ctx.h, nc.FLOAT32, nc.FLOAT64, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE
-    assert call(ctx.h, 10, 10, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE
+    assert call(ctx.h, 12, 12, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE     # past the last dtype
     assert call(ctx.h, nc.INT8, nc.INT32, 4, 5, 3, pa, pb, pc) == nc.ERR_TYPE   # widening needs the flag
     assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 1 << 30, 5, 3, pa, pb, pc) == nc.ERR_SHAPE
     assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, pa, pb, pc, csa=2) == nc.ERR_VALUE
