@@ -187,10 +187,10 @@ def bench_suite(nc, steps, warmup, seed):
     for dt, n, dist, lo, hi in ((nc.FLOAT32, 512, nc.DIST_UNIFORM, -1, 1), (nc.FLOAT64, 2048, nc.DIST_NORMAL, 0, 0),
                                 (nc.INT32, 2048, nc.DIST_INT, -100, 100), (nc.INT8, 4096, nc.DIST_INT, -128, 127)):
         A, B, Cm = ctx.array((n, n), dt), ctx.array((n, n), dt), ctx.array((n, n), dt, zeros=True)
-        A.fill_random(dist, seed, 0, lo, hi)
-        B.fill_random(dist, seed, 1, lo, hi)
         for x in (A, B, Cm):
             x.to_device()
+        A.fill_random(dist, seed, 0, lo, hi)   # device-current arrays fill on the GPU
+        B.fill_random(dist, seed, 1, lo, hi)
         name, peak, fam = peaks[dt]
         t = time_method(nc, ctx, A, B, Cm, nc.METHOD_AUTO, fam, steps, warmup, n)
         flops = 2.0 * n ** 3
@@ -212,10 +212,10 @@ def bench_halves(nc, ctx, steps, warmup, seed):
     n = 8192
     for dt, name, fam in ((nc.BFLOAT16, "bf16", nc.FAMILY_BF16), (nc.FLOAT16, "f16", nc.FAMILY_F16)):
         A, B, Cm = ctx.array((n, n), dt), ctx.array((n, n), dt), ctx.array((n, n), dt, zeros=True)
-        A.fill_random(nc.DIST_NORMAL, seed, 30, 0, 0)
-        B.fill_random(nc.DIST_NORMAL, seed, 31, 0, 0)
         for x in (A, B, Cm):
             x.to_device()
+        A.fill_random(nc.DIST_NORMAL, seed, 30, 0, 0)
+        B.fill_random(nc.DIST_NORMAL, seed, 31, 0, 0)
         t = time_method(nc, ctx, A, B, Cm, nc.METHOD_AUTO, fam, steps, warmup, n)
         flops = 2.0 * n ** 3
         print(json.dumps({"suite": name, "n": n, "ms_per_step": round(t["ms_per_step"], 5),
@@ -248,10 +248,10 @@ def bench_consumers(nc, ctx, steps, warmup, seed):
     M, K, N = 8192, 4096, 4096
     x, w, bias, y = ctx.array((M, K), nc.FLOAT32), ctx.array((K, N), nc.FLOAT32), ctx.array((N,), nc.FLOAT32), \
         ctx.array((M, N), nc.FLOAT32, zeros=True)
-    for i, a in enumerate((x, w, bias)):
-        a.fill_random(nc.DIST_NORMAL, seed, 10 + i, 0, 0)
     for a in (x, w, bias, y):
         a.to_device()
+    for i, a in enumerate((x, w, bias)):
+        a.fill_random(nc.DIST_NORMAL, seed, 10 + i, 0, 0)
     ms = time_calls(nc, ctx, lambda: nc.linear(x, w, bias, y), steps, warmup)
     flops = 2.0 * M * N * K
     print(json.dumps({"suite": "linear_f32", "shape": [M, K, N], "ms_per_step": round(ms, 5),
@@ -260,10 +260,10 @@ def bench_consumers(nc, ctx, steps, warmup, seed):
     B, H, S, D = 8, 16, 1024, 64
     q, kt, s = ctx.array((B, H, S, D), nc.FLOAT32), ctx.array((B, H, D, S), nc.FLOAT32), \
         ctx.array((B, H, S, S), nc.FLOAT32, zeros=True)
-    q.fill_random(nc.DIST_NORMAL, seed, 20, 0, 0)
-    kt.fill_random(nc.DIST_NORMAL, seed, 21, 0, 0)
     for a in (q, kt, s):
         a.to_device()
+    q.fill_random(nc.DIST_NORMAL, seed, 20, 0, 0)
+    kt.fill_random(nc.DIST_NORMAL, seed, 21, 0, 0)
     ms = time_calls(nc, ctx, lambda: nc.bmm(q, kt, s), steps, warmup)
     flops = 2.0 * B * H * S * S * D
     out_bytes = 4.0 * B * H * S * S

@@ -131,7 +131,11 @@ NUMC_API int numc_device_copy(NumcCtx *ctx, void *dst, const void *src, size_t b
  * Fills a contiguous array with the documented generator: element i comes from
  * xoshiro256** block i>>16 seeded by splitmix64 from seed, stream and block.
  * dist: 0 uniform [lo,hi), 1 normal(0,1) (Box-Muller), 2 integer uniform
- * [lo,hi] by rejection.  Host-side; leaves the host copy current. */
+ * [lo,hi] by rejection.  16-bit floats round the binary32 value.  An array
+ * whose device copy is current (numc_array_to_device) fills on the GPU and
+ * stays device-current, with the same values (normal: within one unit in the
+ * last place, a few for float64, where the device's log/sin/cos differ); any
+ * other array fills on the host and leaves the host copy current. */
 NUMC_API int numc_array_fill_random(NumcArray *arr, int dist, uint64_t seed,
                                     uint64_t stream, double lo, double hi);
 

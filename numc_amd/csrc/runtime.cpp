@@ -5,6 +5,7 @@
 // residency state of each storage (both private: NumcCtx and NumcArray are
 // opaque in numc's ABI, include/numc/array.h:10-11).
 #include "internal.h"
+#include "rng.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -794,77 +795,12 @@ extern "C" NUMC_API int numc_device_copy(NumcCtx *ctx, void *dst, const void *sr
 }
 
 // ---------------------------------------------------------------------------
-// seeded inputs: splitmix64-seeded xoshiro256**, one stream per 2^16-element
-// block so the fill is parallel and independent of the thread count.
+// seeded inputs (rng.h): one xoshiro256** stream per 2^16-element block, so
+// the fill is parallel and independent of the thread count.  A device-current
+// array fills on the GPU (generate.hip) with the same code.
 // ---------------------------------------------------------------------------
 
-namespace {
-struct Xoshiro {
-  uint64_t s[4];
-  static uint64_t mix(uint64_t &st) {
-    uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-  }
-  Xoshiro(uint64_t seed, uint64_t stream, uint64_t block) {
-    uint64_t st = seed ^ (stream * 0x9E3779B97F4A7C15ULL) ^ (block * 0xD1B54A32D192ED03ULL);
-    for (auto &w : s) w = mix(st);
-  }
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  uint64_t next() {
-    uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return r;
-  }
-};
-
-template <class T> void put(void *p, size_t i, T v) { ((T *)p)[i] = v; }
-
-// binary32 -> bfloat16 / binary16 bits, round to nearest even (NaN stays NaN)
-uint16_t f32_to_bf16(float x) {
-  uint32_t u;
-  memcpy(&u, &x, 4);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0x7FFFFFu) ? 0x40u : 0u));
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-uint16_t f32_to_f16(float x) {
-  _Float16 h = (_Float16)x; // the compiler's conversion rounds to nearest even
-  uint16_t b;
-  memcpy(&b, &h, 2);
-  return b;
-}
-// double -> binary32, kept as a separate rounding: the optimiser folds
-// (_Float16)(float)d into one double -> half rounding, which differs on ties
-float narrow(double v) {
-  volatile float f = (float)v;
-  return f;
-}
-
-void store_val(void *p, NumcDType dt, size_t i, double fv, int64_t iv, bool is_int) {
-  switch (dt) {
-  case NUMC_DTYPE_INT8: put<int8_t>(p, i, (int8_t)iv); break;
-  case NUMC_DTYPE_INT16: put<int16_t>(p, i, (int16_t)iv); break;
-  case NUMC_DTYPE_INT32: put<int32_t>(p, i, (int32_t)iv); break;
-  case NUMC_DTYPE_INT64: put<int64_t>(p, i, iv); break;
-  case NUMC_DTYPE_UINT8: put<uint8_t>(p, i, (uint8_t)iv); break;
-  case NUMC_DTYPE_UINT16: put<uint16_t>(p, i, (uint16_t)iv); break;
-  case NUMC_DTYPE_UINT32: put<uint32_t>(p, i, (uint32_t)iv); break;
-  case NUMC_DTYPE_UINT64: put<uint64_t>(p, i, (uint64_t)iv); break;
-  case NUMC_DTYPE_FLOAT32: put<float>(p, i, is_int ? (float)iv : (float)fv); break;
-  case NUMC_DTYPE_FLOAT64: put<double>(p, i, is_int ? (double)iv : fv); break;
-  // 16-bit floats: through binary32, then rounded to nearest even
-  case NUMC_DTYPE_BFLOAT16: put<uint16_t>(p, i, f32_to_bf16(is_int ? (float)iv : (float)fv)); break;
-  case NUMC_DTYPE_FLOAT16: put<uint16_t>(p, i, f32_to_f16(narrow(is_int ? (double)iv : fv))); break;
-  }
-}
-} // namespace
+int numc_fill_random_device(NumcArray *arr, const NumcFillSpec &f); // generate.hip
 
 extern "C" NUMC_API int numc_array_fill_random(NumcArray *arr, int dist, uint64_t seed, uint64_t stream,
                                                double lo, double hi) {
@@ -873,47 +809,12 @@ extern "C" NUMC_API int numc_array_fill_random(NumcArray *arr, int dist, uint64_
   if (dist < 0 || dist > 2) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "fill_random: dist %d not in 0..2", dist);
   if (dist == 2 && (hi < lo || hi - lo >= 4294967296.0))
     return NUMC_SET_ERROR(NUMC_ERR_VALUE, "fill_random: integer range must satisfy 0 <= hi-lo < 2^32");
-  host_current(arr);
+  const NumcFillSpec f = numc_fill_spec(arr->dtype, dist, seed, stream, lo, hi);
+  if (arr->st->residency == RES_DEVICE) return numc_fill_random_device(arr, f);
   const size_t n = arr->size, nb = (n + 65535) >> 16;
   void *p = arr->data;
-  NumcDType dt = arr->dtype;
-  const int64_t ilo = (int64_t)lo;
-  const uint64_t span = dist == 2 ? (uint64_t)((int64_t)hi - ilo) + 1 : 1;
-  const uint64_t limit = dist == 2 ? ((1ULL << 32) / span) * span : 0;
   auto work = [&](size_t b0, size_t b1) {
-    for (size_t b = b0; b < b1; b++) {
-      Xoshiro r(seed, stream, b);
-      size_t end = (b + 1) << 16;
-      if (end > n) end = n;
-      for (size_t i = b << 16; i < end;) {
-        if (dist == 0) {
-          uint64_t x = r.next();
-          if (dt == NUMC_DTYPE_FLOAT32 || dt == NUMC_DTYPE_BFLOAT16 || dt == NUMC_DTYPE_FLOAT16) {
-            float u = (float)(x >> 40) * 0x1p-24f;
-            float v = (float)lo + ((float)hi - (float)lo) * u;
-            if (dt == NUMC_DTYPE_FLOAT32) put<float>(p, i, v);
-            else put<uint16_t>(p, i, dt == NUMC_DTYPE_BFLOAT16 ? f32_to_bf16(v) : f32_to_f16(v));
-          } else {
-            double u = (double)(x >> 11) * 0x1p-53;
-            store_val(p, dt, i, lo + (hi - lo) * u, 0, false);
-          }
-          i++;
-        } else if (dist == 1) {
-          uint64_t x = r.next(), y = r.next();
-          double u1 = (double)((x >> 11) + 1) * 0x1p-53, u2 = (double)(y >> 11) * 0x1p-53;
-          double rad = std::sqrt(-2.0 * std::log(u1)), th = 6.283185307179586476925286766559 * u2;
-          store_val(p, dt, i, rad * std::cos(th), 0, false);
-          if (i + 1 < end) store_val(p, dt, i + 1, rad * std::sin(th), 0, false);
-          i += 2;
-        } else {
-          uint64_t x32;
-          do x32 = r.next() >> 32;
-          while (x32 >= limit);
-          store_val(p, dt, i, 0.0, ilo + (int64_t)(x32 % span), true);
-          i++;
-        }
-      }
-    }
+    for (size_t b = b0; b < b1; b++) numc_fill_block(p, n, b, f);
   };
   unsigned nt = std::thread::hardware_concurrency();
   if (nt > 16) nt = 16;

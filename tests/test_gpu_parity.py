@@ -379,6 +379,41 @@ def test_device_residency(ctx):
     assert np.all(O.numpy() == 0)
 
 
+FILL_CASES = [(nc.FLOAT32, nc.DIST_UNIFORM, -2, 3), (nc.FLOAT64, nc.DIST_UNIFORM, -1, 1),
+              (nc.INT32, nc.DIST_INT, -100, 100), (nc.INT8, nc.DIST_INT, -128, 127),
+              (nc.UINT16, nc.DIST_INT, 0, 65535), (nc.BFLOAT16, nc.DIST_UNIFORM, -1, 1),
+              (nc.FLOAT16, nc.DIST_UNIFORM, -1, 1), (nc.FLOAT32, nc.DIST_NORMAL, 0, 0),
+              (nc.FLOAT64, nc.DIST_NORMAL, 0, 0), (nc.BFLOAT16, nc.DIST_NORMAL, 0, 0),
+              (nc.FLOAT16, nc.DIST_NORMAL, 0, 0)]
+
+
+@pytest.mark.parametrize("case", FILL_CASES, ids=lambda c: "%d-%d" % c[:2])
+def test_device_fill_matches_host_fill(ctx, case):
+    """a device-current array fills on the GPU with the host fill's code:
+    identical uniform and integer values; normal values within one unit in
+    the last place of the stored type, eight for float64 (the device computes
+    log/sin/cos with its own binary64 library); the array stays device-current"""
+    dt, dist, lo, hi = case
+    n = 3 * 65536 + 1234                       # four streams, the last one partial
+    D, H = ctx.array((n,), dt), ctx.array((n,), dt)
+    D.to_device()
+    D.fill_random(dist, 77, 4, lo, hi)
+    assert nc.lib().numc_array_on_device(D.h)
+    H.fill_random(dist, 77, 4, lo, hi)
+    d, h = D.numpy(), H.numpy()
+    if dist != nc.DIST_NORMAL:
+        assert np.array_equal(d, h)
+        return
+    bits = {2: np.int16, 4: np.int32, 8: np.int64}[d.dtype.itemsize]
+    di, hi_ = d.view(bits).astype(np.int64), h.view(bits).astype(np.int64)
+    # float64 shows the libraries' differences directly (log, cos/sin near
+    # their zeros, and the product: a few units in the last place); narrower
+    # types round them away except next to a rounding boundary
+    f64 = dt == nc.FLOAT64
+    assert np.all(np.abs(di - hi_) <= (8 if f64 else 1))   # units in the last place
+    assert np.mean(di == hi_) > (0.5 if f64 else 0.999)
+
+
 def test_async_mode(ctx):
     ctx.set_sync(False)
     a = np.ones((64, 64), np.float32)
