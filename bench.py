@@ -305,13 +305,14 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--only", default=None, help="time one method: split_bf16, f32 or fp8")
     ap.add_argument("--suite", action="store_true",
-                    help="time the other reference configurations (f32 512^3, f64 2048^3, int32 2048^3, int8 4096^3)")
+                    help="time the other configurations (f32 512^3, f64 2048^3, int32 2048^3, int8 4096^3, "
+                         "bf16/f16 8192^3, numc_linear, numc_bmm)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the reference copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
-              "'How it would shard'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (one context drives "
+              "one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numc_amd as nc
