@@ -63,8 +63,10 @@ NUMC_API int numc_linear_ex(const NumcArray *x, const NumcArray *w, const NumcAr
  * of numc's roadmap (ROADMAP.md:140; numc has no implementation).  a is
  * (..., M, K), b (..., K, N), both at least 2-D; the batch axes broadcast as
  * numpy.matmul's do (aligned from the right, equal or 1) and out is
- * (batch..., M, N).  All slices run in one persistent numc_matmul_batched
- * launch (a single slice runs as numc_matmul). */
+ * (batch..., M, N).  When each operand's slices sit at offsets linear in the
+ * batch index (or the operand is 2-D), all slices run as one strided-batch
+ * GEMM launch; other layouts run in one persistent numc_matmul_batched launch
+ * (a single slice runs as numc_matmul). */
 NUMC_API int numc_bmm(const NumcArray *a, const NumcArray *b, NumcArray *out);
 
 /* Pointer-level GEMM for a host library that keeps its own arrays: the call

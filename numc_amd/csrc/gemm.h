@@ -20,6 +20,10 @@ struct GemmProblem {
   long long rso, cso;
   const void *bias;      // optional: out[i][j] += bias[j * bias_s] (output dtype), fused into the store
   long long bias_s;
+  // strided batch (0 or 1: one problem): slice z reads A + z*a_bs, B + z*b_bs
+  // and writes C + z*c_bs (elements; 0 = the same matrix for every slice)
+  int batch;
+  long long a_bs, b_bs, c_bs;
 };
 
 struct GemmArgs;
@@ -43,6 +47,7 @@ struct PackArgs {
   int *flags;         // integer digit planes: nonzero mask
   int mode;           // batched pack: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 lane slots,
                       // 3 int8 byte tiles (k-contiguous), 4 int8 byte tiles (row-contiguous)
+  long long src_zs, dst_zs; // strided batch: bytes between slices of the source and of the packed planes
 };
 
 // one problem of the persistent batched kernel (packed operands in a workspace)
@@ -69,6 +74,8 @@ int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix
                       hipStream_t st);
 
 const GemmConfig *numc_gemm_config(int family, int idx);
+// configuration for a strided batch of equal M x N slices
+const GemmConfig *numc_strided_batch_config(int family, int M, int N);
 int numc_gemm_config_count(int family);
 // device bytes needed in ctx->ws (and ctx->ws_aux) for a problem under a config
 size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *aux_bytes);

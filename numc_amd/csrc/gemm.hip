@@ -136,12 +136,20 @@ __device__ inline void pack_unit_i8(const PackArgs &a, long long t) {
   *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// strided batch: blockIdx.z selects the slice (a no-op for one problem)
+__device__ __attribute__((always_inline)) inline void zslice(PackArgs &a) {
+  a.src = (const unsigned char *)a.src + (long long)blockIdx.z * a.src_zs;
+  a.dst += (long long)blockIdx.z * a.dst_zs;
+}
+
 __global__ void k_pack_f32(PackArgs a) {
+  zslice(a);
   long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t < (long long)a.nRT * a.nKF * 64) pack_unit_f32(a, t);
 }
 
 __global__ void k_pack_split_bf16(PackArgs a) {
+  zslice(a);
   long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t < (long long)a.nRT * a.nKF * 64) pack_unit_split(a, t);
 }
@@ -149,6 +157,7 @@ __global__ void k_pack_split_bf16(PackArgs a) {
 
 // f32 * 2^row_exp[row] -> e4m3  (T=32, 16 per lane, KF=32)
 __global__ void k_pack_fp8(PackArgs a) {
+  zslice(a);
   long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t >= (long long)a.nRT * a.nKF * 64) return;
   int lane = (int)(t & 63);
@@ -263,6 +272,7 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
 
 template <int MODE_PK>
 __global__ void __launch_bounds__(256) k_pack_f32_tiled(PackArgs a) {
+  zslice(a);
   __shared__ float tile[PACK_TILE_WORDS];
   pack_f32_tile_k<MODE_PK>(a, blockIdx.x, tile);
 }
@@ -363,12 +373,14 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
 
 template <int MODE_PK>
 __global__ void __launch_bounds__(256) k_pack_f32_tiled_rows(PackArgs a) {
+  zslice(a);
   __shared__ float tile[PACK_TILE_WORDS];
   pack_f32_tile_rows<MODE_PK>(a, blockIdx.x, tile);
 }
 
 // f64 -> f64 (T=16, 2 per lane, KF=8)
 __global__ void k_pack_f64(PackArgs a) {
+  zslice(a);
   long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t >= (long long)a.nRT * a.nKF * 64) return;
   int lane = (int)(t & 63);
@@ -402,6 +414,7 @@ __device__ inline void flags_or(int *flags, int mask) {
 
 template <class S, int P>
 __global__ void k_pack_digits(PackArgs a) {
+  zslice(a);
   long long t = blockIdx.x * 256LL + threadIdx.x;
   bool live = t < (long long)a.nRT * a.nKF * 64;
   uint32_t w[P][4];
@@ -796,7 +809,10 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
 
 template <class MF, int BM, int BN, int KC, int WGM, int WGN, int MODE, int EPI, int NS, int VAR>
 __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
-  const GemmArgs g = gin; // local copy: the tile code captures it by reference
+  GemmArgs g = gin; // local copy: the tile code captures it by reference
+  g.A += (long long)blockIdx.z * g.a_zs; // strided batch: slice blockIdx.z
+  g.B += (long long)blockIdx.z * g.b_zs;
+  g.C = (unsigned char *)g.C + (long long)blockIdx.z * g.c_zs;
   // NS stages of fragments, filled straight from HBM by global_load_lds
   __shared__ u32x4 lds[TileShape<MF, BM, BN, KC, WGM, WGN, MODE, NS>::LDS_VEC];
   // block -> tile: blocks with equal pid % 8 share an XCD, so give each XCD
@@ -1043,6 +1059,24 @@ static const GemmConfig kF16[] = {
     CFGB(MfF16, 64, 64, 64, 2, 2, EPI_F16, 8, 4),
 };
 
+// strided batches of equal slices (numc_bmm): small tiles with two LDS
+// stages, so that several blocks share a CU and one block's copies and
+// stores overlap another's MFMAs (no split-K: the slices fill the GPU)
+static const GemmConfig kStridedBatch[NUMC_FAMILY_COUNT][2] = {
+    {CFGB(MfF32, 128, 128, 16, 2, 2, EPI_F32, 1, 2), CFGB(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 2)},
+    {CFGS(MfBF16, 128, 128, 16, 2, 2, MODE_SPLIT3, EPI_F32, 1, 2), CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 2)},
+    {CFGS(MfFP8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 2), CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 2)},
+    {CFGB(MfF64, 64, 64, 16, 2, 2, EPI_F64, 1, 2), CFGB(MfF64, 64, 64, 32, 2, 2, EPI_F64, 1, 2)},
+    {CFGB(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 2), CFGB(MfI8, 64, 64, 64, 2, 2, EPI_INT, 1, 2)},
+    {CFGB(MfBF16, 128, 128, 32, 2, 2, EPI_BF16, 1, 2), CFGB(MfBF16, 64, 64, 64, 2, 2, EPI_BF16, 1, 2)},
+    {CFGB(MfF16, 128, 128, 32, 2, 2, EPI_F16, 1, 2), CFGB(MfF16, 64, 64, 64, 2, 2, EPI_F16, 1, 2)},
+};
+
+const GemmConfig *numc_strided_batch_config(int family, int M, int N) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT) return nullptr;
+  return &kStridedBatch[family][M >= 128 && N >= 128 ? 0 : 1];
+}
+
 #undef CFGB
 
 struct FamilyTable {
@@ -1217,6 +1251,7 @@ __device__ __attribute__((always_inline)) inline void digits_tile_k(const PackAr
 
 template <class S, int P>
 __global__ void __launch_bounds__(256) k_pack_digits_tiled(PackArgs a) {
+  zslice(a);
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   digits_tile_k<S, P>(a, blockIdx.x, tile);
 }
@@ -1275,6 +1310,7 @@ __device__ __attribute__((always_inline)) inline void digits_tile_rows(const Pac
 
 template <class S, int P>
 __global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
+  zslice(a);
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   digits_tile_rows<S, P>(a, blockIdx.x, tile);
 }
@@ -1341,6 +1377,7 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArg
 }
 
 __global__ void __launch_bounds__(256) k_pack_bytes_k(PackArgs a) {
+  zslice(a);
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   bytes_tile_k(a, blockIdx.x, tile);
 }
@@ -1417,6 +1454,7 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_rows(const Pack
 }
 
 __global__ void __launch_bounds__(256) k_pack_bytes_rows(PackArgs a) {
+  zslice(a);
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   bytes_tile_rows(a, blockIdx.x, tile);
 }
@@ -1429,6 +1467,7 @@ __global__ void __launch_bounds__(256) k_pack_bytes_rows(PackArgs a) {
 
 // k-contiguous rows: the int8 byte tile with K and the row stride in bytes
 __global__ void __launch_bounds__(256) k_pack_raw16_k(PackArgs a) {
+  zslice(a);
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   a.K *= 2;
   a.rs *= 2;
@@ -1498,12 +1537,14 @@ __device__ __attribute__((always_inline)) inline void raw16_tile_rows(const Pack
 }
 
 __global__ void __launch_bounds__(256) k_pack_raw16_rows(PackArgs a) {
+  zslice(a);
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   raw16_tile_rows(a, blockIdx.x, tile);
 }
 
 // any other strides: one thread per fragment lane slot
 __global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
+  zslice(a);
   const long long t = blockIdx.x * 256LL + threadIdx.x;
   if (t >= (long long)a.nRT * a.nKF * 64) return;
   const int lane = (int)(t & 63);
@@ -1590,45 +1631,45 @@ int numc_batch_pack_blocks(int family, PackArgs &pk) {
   return (int)cdiv((long long)pk.nRT * pk.nKF * 64, 256);
 }
 
-template <class S, int P> static void launch_digits_p(PackArgs &pk, unsigned nb, hipStream_t st) {
+template <class S, int P> static void launch_digits_p(PackArgs &pk, unsigned nb, unsigned nz, hipStream_t st) {
   if constexpr (sizeof(S) == 1 && P == 1) {
     if (pk.ks == 1) {
-      hipLaunchKernelGGL(k_pack_bytes_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16))), dim3(256), 0, st, pk);
+      hipLaunchKernelGGL(k_pack_bytes_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16)), 1, nz), dim3(256), 0, st, pk);
       return;
     }
     if (pk.rs == 1) {
-      dim3 grid((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4)));
-      hipLaunchKernelGGL(k_pack_bytes_rows, dim3(grid), dim3(256), 0, st, pk);
+      dim3 grid((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4)), 1, nz);
+      hipLaunchKernelGGL(k_pack_bytes_rows, grid, dim3(256), 0, st, pk);
       return;
     }
   }
   if (pk.ks == 1) {
-    dim3 grid((unsigned)((long long)pk.nRT * cdiv((long long)pk.nKF * 32, 128)));
+    dim3 grid((unsigned)((long long)pk.nRT * cdiv((long long)pk.nKF * 32, 128)), 1, nz);
     hipLaunchKernelGGL((k_pack_digits_tiled<S, P>), grid, dim3(256), 0, st, pk);
   } else if (pk.rs == 1) {
-    dim3 grid((unsigned)(cdiv(pk.nRT, 4) * pk.nKF));
+    dim3 grid((unsigned)(cdiv(pk.nRT, 4) * pk.nKF), 1, nz);
     hipLaunchKernelGGL((k_pack_digits_tiled_rows<S, P>), grid, dim3(256), 0, st, pk);
   } else {
-    hipLaunchKernelGGL((k_pack_digits<S, P>), dim3(nb), dim3(256), 0, st, pk);
+    hipLaunchKernelGGL((k_pack_digits<S, P>), dim3(nb, 1, nz), dim3(256), 0, st, pk);
   }
 }
 
 template <class S>
-static void launch_digits(int planes, PackArgs &pk, unsigned nb, hipStream_t st) {
+static void launch_digits(int planes, PackArgs &pk, unsigned nb, unsigned nz, hipStream_t st) {
   switch (planes) {
-  case 1: launch_digits_p<S, 1>(pk, nb, st); break;
-  case 2: launch_digits_p<S, 2>(pk, nb, st); break;
-  default: launch_digits_p<S, 4>(pk, nb, st); break;
+  case 1: launch_digits_p<S, 1>(pk, nb, nz, st); break;
+  case 2: launch_digits_p<S, 2>(pk, nb, nz, st); break;
+  default: launch_digits_p<S, 4>(pk, nb, nz, st); break;
   }
 }
 
-static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, hipStream_t st) {
+static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, unsigned nz, hipStream_t st) {
   unsigned nb = blocks_for((long long)pk.nRT * pk.nKF * 64);
   if (pk.ks == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_FP8)) {
     // rows with unit k-stride: stage 32 x 128 tiles through LDS
     const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
     const long long nkt = cdiv((long long)pk.nKF * kf, 128);
-    dim3 grid((unsigned)((long long)pk.nRT * nkt));
+    dim3 grid((unsigned)((long long)pk.nRT * nkt), 1, nz);
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_f32_tiled<0>), grid, dim3(256), 0, st, pk);
     else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_f32_tiled<1>), grid, dim3(256), 0, st, pk);
     else hipLaunchKernelGGL((k_pack_f32_tiled<2>), grid, dim3(256), 0, st, pk);
@@ -1637,7 +1678,7 @@ static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, h
   if (pk.rs == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_FP8)) {
     // rows contiguous: stage 128 x 32 tiles through LDS
     const long long nkt = cdiv((long long)pk.nKF * family_kf(family), 32);
-    dim3 grid((unsigned)(cdiv(pk.nRT, 4) * nkt));
+    dim3 grid((unsigned)(cdiv(pk.nRT, 4) * nkt), 1, nz);
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_f32_tiled_rows<0>), grid, dim3(256), 0, st, pk);
     else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_f32_tiled_rows<1>), grid, dim3(256), 0, st, pk);
     else hipLaunchKernelGGL((k_pack_f32_tiled_rows<2>), grid, dim3(256), 0, st, pk);
@@ -1645,26 +1686,26 @@ static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, h
   }
   if (family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16) { // raw 16-bit copies
     if (pk.ks == 1)
-      hipLaunchKernelGGL(k_pack_raw16_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16))), dim3(256), 0, st, pk);
+      hipLaunchKernelGGL(k_pack_raw16_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16)), 1, nz), dim3(256), 0, st, pk);
     else if (pk.rs == 1)
-      hipLaunchKernelGGL(k_pack_raw16_rows, dim3((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4))), dim3(256), 0, st, pk);
+      hipLaunchKernelGGL(k_pack_raw16_rows, dim3((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4)), 1, nz), dim3(256), 0, st, pk);
     else
-      hipLaunchKernelGGL(k_pack_raw16, dim3(nb), dim3(256), 0, st, pk);
+      hipLaunchKernelGGL(k_pack_raw16, dim3(nb, 1, nz), dim3(256), 0, st, pk);
     return;
   }
   switch (family) {
-  case NUMC_FAMILY_F32: hipLaunchKernelGGL(k_pack_f32, dim3(nb), dim3(256), 0, st, pk); break;
-  case NUMC_FAMILY_SPLIT_BF16: hipLaunchKernelGGL(k_pack_split_bf16, dim3(nb), dim3(256), 0, st, pk); break;
-  case NUMC_FAMILY_FP8: hipLaunchKernelGGL(k_pack_fp8, dim3(nb), dim3(256), 0, st, pk); break;
-  case NUMC_FAMILY_F64: hipLaunchKernelGGL(k_pack_f64, dim3(nb), dim3(256), 0, st, pk); break;
+  case NUMC_FAMILY_F32: hipLaunchKernelGGL(k_pack_f32, dim3(nb, 1, nz), dim3(256), 0, st, pk); break;
+  case NUMC_FAMILY_SPLIT_BF16: hipLaunchKernelGGL(k_pack_split_bf16, dim3(nb, 1, nz), dim3(256), 0, st, pk); break;
+  case NUMC_FAMILY_FP8: hipLaunchKernelGGL(k_pack_fp8, dim3(nb, 1, nz), dim3(256), 0, st, pk); break;
+  case NUMC_FAMILY_F64: hipLaunchKernelGGL(k_pack_f64, dim3(nb, 1, nz), dim3(256), 0, st, pk); break;
   default:
     switch (in_dt) {
-    case NUMC_DTYPE_INT8: launch_digits<int8_t>(planes, pk, nb, st); break;
-    case NUMC_DTYPE_UINT8: launch_digits<uint8_t>(planes, pk, nb, st); break;
-    case NUMC_DTYPE_INT16: launch_digits<int16_t>(planes, pk, nb, st); break;
-    case NUMC_DTYPE_UINT16: launch_digits<uint16_t>(planes, pk, nb, st); break;
-    case NUMC_DTYPE_INT32: launch_digits<int32_t>(planes, pk, nb, st); break;
-    default: launch_digits<uint32_t>(planes, pk, nb, st); break;
+    case NUMC_DTYPE_INT8: launch_digits<int8_t>(planes, pk, nb, nz, st); break;
+    case NUMC_DTYPE_UINT8: launch_digits<uint8_t>(planes, pk, nb, nz, st); break;
+    case NUMC_DTYPE_INT16: launch_digits<int16_t>(planes, pk, nb, nz, st); break;
+    case NUMC_DTYPE_UINT16: launch_digits<uint16_t>(planes, pk, nb, nz, st); break;
+    case NUMC_DTYPE_INT32: launch_digits<int32_t>(planes, pk, nb, nz, st); break;
+    default: launch_digits<uint32_t>(planes, pk, nb, nz, st); break;
     }
   }
 }
@@ -1699,7 +1740,8 @@ size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *au
   int planes = p.family == NUMC_FAMILY_SPLIT_BF16 ? 2 : (p.family == NUMC_FAMILY_I8 ? digit_planes(p.out_dt) : 1);
   size_t raw = (p.family == NUMC_FAMILY_F64) ? 8 : 4;
   size_t partial = c.cfg.split_k > 1 ? (size_t)c.cfg.split_k * p.M * p.N * raw : 0;
-  size_t bytes = (size_t)(pa + pb) * planes + partial + 3 * 256;
+  const size_t nz = p.batch > 1 ? (size_t)p.batch : 1;
+  size_t bytes = (size_t)(pa + pb) * planes * nz + partial + 3 * 256;
   if (aux_bytes) *aux_bytes = 2 * NUMC_FLAG_INTS * sizeof(int) + (size_t)(p.M + p.N) * 8 + 512;
   return bytes;
 }
@@ -1715,8 +1757,10 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   if (p.M == 0 || p.N == 0) return 0;
   if (p.K == 0) {
     long long n = (long long)p.M * p.N;
-    hipLaunchKernelGGL(k_zero_strided, dim3(blocks_for(n)), dim3(256), 0, st, (unsigned char *)p.C, p.M, p.N,
-                       p.rso, p.cso, (int)numc_dtype_size(p.out_dt), (const unsigned char *)p.bias, p.bias_s);
+    const int es = (int)numc_dtype_size(p.out_dt);
+    for (int z = 0; z < (p.batch > 1 ? p.batch : 1); z++)
+      hipLaunchKernelGGL(k_zero_strided, dim3(blocks_for(n)), dim3(256), 0, st, (unsigned char *)p.C + z * p.c_bs * es,
+                         p.M, p.N, p.rso, p.cso, es, (const unsigned char *)p.bias, p.bias_s);
     return 0;
   }
 
@@ -1724,12 +1768,15 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   long long pla, plb;
   plane_geometry(c, p.M, p.K, kf, c.cfg.bm, &nRTa, &nKF, &pla);
   plane_geometry(c, p.N, p.K, kf, c.cfg.bn, &nRTb, &nKF, &plb);
+  // strided batch: slice z's packed planes follow slice z-1's
+  const int nz = p.batch > 1 ? p.batch : 1;
+  if (nz > 1 && (c.cfg.split_k > 1 || family == NUMC_FAMILY_FP8 || nz > 65535))
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: a strided batch needs a configuration without split-K");
   unsigned char *wsb = (unsigned char *)ws;
   unsigned char *Ap = wsb;
-  unsigned char *Bp = Ap + pla * planes;
-  size_t raw = family == NUMC_FAMILY_F64 ? 8 : 4;
+  unsigned char *Bp = Ap + pla * planes * nz;
   void *partial = c.cfg.split_k > 1 ? (void *)(Bp + plb * planes) : nullptr;
-  (void)raw;
+  const size_t ein = numc_dtype_size(p.in_dt), eout = numc_dtype_size(p.out_dt);
 
   // aux: the digit plane masks (NUMC_FLAG_INTS ints per operand), then FP8
   // row max bits / exponents
@@ -1754,13 +1801,15 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   pk.src = p.A; pk.rs = p.rsa; pk.ks = p.csa; pk.R = p.M; pk.K = p.K;
   pk.nRT = nRTa; pk.nKF = nKF; pk.dst = Ap; pk.plane_bytes = pla;
   pk.row_exp = expA; pk.flags = flags;
+  pk.src_zs = p.a_bs * (long long)ein; pk.dst_zs = pla * planes;
   numc_prof_begin(ctx, PROF_PACK, family);
-  launch_pack(family, p.in_dt, planes, pk, st);
+  launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
   // B^T: rows N, element (j, k) at B[k*rsb + j*csb]
   pk.src = p.B; pk.rs = p.csb; pk.ks = p.rsb; pk.R = p.N; pk.K = p.K;
   pk.nRT = nRTb; pk.nKF = nKF; pk.dst = Bp; pk.plane_bytes = plb;
   pk.row_exp = expB; pk.flags = flags + NUMC_FLAG_INTS;
-  launch_pack(family, p.in_dt, planes, pk, st);
+  pk.src_zs = p.b_bs * (long long)ein; pk.dst_zs = plb * planes;
+  launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
   numc_prof_end(ctx);
 
   GemmArgs g{};
@@ -1786,7 +1835,10 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.pa[0] = g.pb[0] = 0;
   g.bias = p.bias;
   g.bias_s = p.bias_s;
-  dim3 grid((unsigned)(g.nbm * g.nbn), (unsigned)g.ksplit);
+  g.a_zs = pla * planes;
+  g.b_zs = plb * planes;
+  g.c_zs = p.c_bs * (long long)eout;
+  dim3 grid((unsigned)(g.nbm * g.nbn), (unsigned)g.ksplit, (unsigned)nz);
   const bool bias = p.bias != nullptr;
   const int passes = family == NUMC_FAMILY_I8 ? planes : 1;
   for (int s = 0; s < passes; s++) {

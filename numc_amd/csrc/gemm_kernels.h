@@ -160,6 +160,7 @@ struct GemmArgs {
   int pa[4], pb[4];           // plane of A / B for each product
   const void *bias;           // optional per-column bias (output dtype), added at the store
   long long bias_s;           // bias stride in elements
+  long long a_zs, b_zs, c_zs; // strided batch: bytes between slices (blockIdx.z) of A, B (packed) and C
 };
 
 // Nonzero-digit-plane masks of the integer packs: each operand's mask is

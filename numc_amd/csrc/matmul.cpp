@@ -690,6 +690,78 @@ static int check_bmm(const NumcArray *a, const NumcArray *b, const NumcArray *ou
   return 0;
 }
 
+// The slice offset (elements) of operand x for the flattened batch index t of
+// out, when it is t * S for every t (S = 0: one matrix for every slice).
+// False when x's batch axes do not follow out's linearly (an axis
+// broadcast inside a batched operand, or gaps between slices).
+static bool batch_stride(const NumcArray *x, const NumcArray *out, long long *S) {
+  const size_t nbo = out->dim - 2;
+  long long inner = 1, s = 0;
+  bool have = false;
+  for (size_t i = nbo; i-- > 0;) {
+    const long long n = (long long)out->shape[i];
+    long long xs = 0;
+    if (i + x->dim >= out->dim) {
+      const size_t xi = i + x->dim - out->dim;
+      if (x->shape[xi] != 1) xs = (long long)(x->strides[xi] / x->elem_size);
+    }
+    if (n > 1) {
+      if (!have) {
+        if (xs % inner) return false;
+        s = xs / inner;
+        have = true;
+      } else if (xs != s * inner) {
+        return false;
+      }
+    }
+    inner *= n;
+  }
+  *S = s;
+  return true;
+}
+
+// every slice of numc_bmm in one strided-batch GEMM (packs and MFMA kernel
+// with blockIdx.z = slice); returns 1 when the operands do not qualify
+static int bmm_strided(const NumcArray *a, const NumcArray *b, NumcArray *out, size_t count) {
+  long long sa, sb, so;
+  if (count < 2 || count > 65535 || !batch_stride(a, out, &sa) || !batch_stride(b, out, &sb) ||
+      !batch_stride(out, out, &so) || so == 0)
+    return 1;
+  const int family = pick_family(a, nullptr);
+  if (family < 0 || family == 100 || family == NUMC_FAMILY_FP8 || family == NUMC_FAMILY_SPLIT_BF16) return 1;
+  const size_t es = a->elem_size;
+  GemmProblem p{};
+  p.family = family;
+  p.in_dt = a->dtype;
+  p.out_dt = out->dtype;
+  p.M = (int)a->shape[a->dim - 2];
+  p.K = (int)a->shape[a->dim - 1];
+  p.N = (int)b->shape[b->dim - 1];
+  if (p.K == 0) return 1;
+  p.rsa = es_stride(a->strides[a->dim - 2], es);
+  p.csa = es_stride(a->strides[a->dim - 1], es);
+  p.rsb = es_stride(b->strides[b->dim - 2], es);
+  p.csb = es_stride(b->strides[b->dim - 1], es);
+  p.rso = es_stride(out->strides[out->dim - 2], out->elem_size);
+  p.cso = es_stride(out->strides[out->dim - 1], out->elem_size);
+  p.batch = (int)count;
+  p.a_bs = sa;
+  p.b_bs = sb;
+  p.c_bs = so;
+  const GemmConfig *cfg = numc_strided_batch_config(family, p.M, p.N);
+  if (!cfg) return 1;
+  NumcCtx *ctx = out->ctx;
+  int err;
+  if ((err = numc_gpu_init(ctx))) return err;
+  if ((err = stage_in(ctx, a, &p.A)) || (err = stage_in(ctx, b, &p.B)) || (err = stage_out_begin(ctx, out, &p.C)))
+    return err;
+  size_t aux = 0, need = numc_gemm_workspace(p, *cfg, &aux);
+  if ((err = numc_ws_reserve(ctx, &ctx->ws, need)) || (err = numc_ws_reserve(ctx, &ctx->ws_aux, aux))) return err;
+  if ((err = numc_gemm_launch(ctx, p, *cfg, ctx->ws.ptr, ctx->ws_aux.ptr))) return err;
+  if ((err = stage_out_end(ctx, out, p.C))) return err;
+  return numc_finish(ctx);
+}
+
 extern "C" NUMC_API int numc_bmm(const NumcArray *a, const NumcArray *b, NumcArray *out) {
   int err = check_bmm(a, b, out);
   if (err) return err;
@@ -698,6 +770,8 @@ extern "C" NUMC_API int numc_bmm(const NumcArray *a, const NumcArray *b, NumcArr
   size_t count = 1;
   for (size_t i = 0; i < nb; i++) count *= out->shape[i];
   if (count == 0 || M == 0 || N == 0) return 0;
+  // equal slices at linear offsets: one strided-batch launch
+  if ((err = bmm_strided(a, b, out, count)) != 1) return err;
   std::vector<NumcArray> views(3 * count);
   std::vector<NumcArray *> pa(count), pb(count), po(count);
   for (size_t t = 0; t < count; t++) {

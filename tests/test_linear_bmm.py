@@ -242,3 +242,34 @@ def test_linear_method_rejected(ctx):
     x, w, o = _arr(ctx, (2, 3)), _arr(ctx, (3, 4)), _arr(ctx, (2, 4))
     assert nc.linear(x, w, None, o, nc.METHOD_SPLIT_BF16) == nc.ERR_VALUE
     assert "linear:" in nc.last_error()[1]
+
+
+@pytest.mark.gpu
+def test_bmm_zero_k_and_many_slices(ctx):
+    """K = 0 gives zeros in every slice (persistent batched kernel); 300
+    small int8 slices wrap exactly"""
+    O = ctx.from_numpy(np.full((6, 5, 7), 9, np.float32))
+    nc.check(nc.bmm(ctx.array((6, 5, 0), nc.FLOAT32), ctx.array((6, 0, 7), nc.FLOAT32), O), "bmm k=0")
+    assert np.all(O.numpy() == 0)
+    rng = np.random.default_rng(11)
+    a = rng.integers(-128, 127, (300, 9, 40), dtype=np.int8, endpoint=True)
+    b = rng.integers(-128, 127, (300, 40, 11), dtype=np.int8, endpoint=True)
+    O = ctx.array((300, 9, 11), nc.INT8, zeros=True)
+    nc.check(nc.bmm(ctx.from_numpy(a), ctx.from_numpy(b), O), "bmm int8")
+    assert np.array_equal(O.numpy(), np.matmul(a, b))   # numpy accumulates int8 mod 2^8
+
+
+@pytest.mark.gpu
+def test_bmm_transposed_a_and_strided_out(ctx):
+    """a given as a transposed view, the output a view with gaps"""
+    rng = np.random.default_rng(12)
+    at = rng.standard_normal((4, 30, 20)).astype(np.float32)        # (batch, K, M) storage
+    b = rng.standard_normal((4, 30, 25)).astype(np.float32)
+    A = ctx.from_numpy(at).transpose((0, 2, 1))                     # (4, 20, 30)
+    O_full = ctx.from_numpy(np.full((4, 20, 27), 5, np.float32))
+    O = O_full.slice(2, 1, 26)                                      # (4, 20, 25)
+    nc.check(nc.bmm(A, ctx.from_numpy(b), O), "bmm views")
+    full = O_full.numpy()
+    want, den = oracle.bmm(np.ascontiguousarray(np.swapaxes(at, 1, 2)), b)
+    _check(full[:, :, 1:26], want, den, 30, np.float32, False)
+    assert np.all(full[:, :, 0] == 5) and np.all(full[:, :, 26] == 5)
