@@ -398,15 +398,14 @@ __global__ void k_pack_f64(PackArgs a) {
 }
 
 // integer -> P signed base-256 digit planes (T=32, 16 per lane, KF=32).
-// x = sum_p d_p 256^p (mod 256^P) with d_p in [-128, 127]; flags[0] gets a
-// bit per plane that holds a nonzero digit.
-// OR a wave's nonzero-plane mask into the shared flags word.  Thousands of
-// waves target the one word, and same-address atomics serialise in L2, so
-// the word is read first (from L2) and the atomic issued only when it would
-// set a new bit; a stale read only costs a redundant atomic.
+// x = sum_p d_p 256^p (mod 256^P) with d_p in [-128, 127]; the operand's
+// mask gets a bit per plane that holds a nonzero digit.
+// OR a wave's nonzero-plane mask into its block's slot of the mask (one of
+// NUMC_FLAG_SLOTS words on separate lines: same-address atomics serialise
+// in L2).  The slot is read first and the atomic issued only when it would
+// set a new bit, so once the bits are set the remaining waves only read; a
+// stale read only costs a redundant atomic.
 __device__ inline void flags_or(int *flags, int mask) {
-  // this block's slot; the load first, so that once the bits are set the
-  // remaining waves only read
   int *slot = flags + (blockIdx.x % NUMC_FLAG_SLOTS) * NUMC_FLAG_STRIDE;
   const int cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (mask & ~cur) atomicOr(slot, mask);
