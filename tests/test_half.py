@@ -177,6 +177,13 @@ def test_dot_linear_bmm(ctx, dt):
     for i in range(2):
         for h in range(3):
             check_gemm(got[i, h], q[i, h], k[0, h], dt, 40)
+    # equal slices at linear offsets (the strided-batch launch), a shared 2-D b
+    q, k = rand16(rng, (4, 130, 72), dt), rand16(rng, (72, 140), dt)
+    O = ctx.array((4, 130, 140), dt, zeros=True)
+    nc.check(nc.bmm(ctx.from_numpy(q, dt), ctx.from_numpy(k, dt), O), "bmm strided")
+    got = O.numpy()
+    for i in range(4):
+        check_gemm(got[i], q[i], k, dt, 72)
 
 
 @pytest.mark.gpu
