@@ -78,6 +78,22 @@ __global__ void k_zero_strided(unsigned char *c, int M, int N, long long rso, lo
 
 template <class S> __device__ inline S ld(const void *p, long long i) { return ((const S *)p)[i]; }
 
+// 16 bytes (16 / sizeof(S) values) of a chunk of an edge tile, zero past
+// n_ok values: one vector load when the chunk is whole and aligned, else
+// element loads (volatile keeps the compiler from merging them with the
+// interior tiles' vector loads)
+template <class S>
+__device__ __attribute__((always_inline)) inline uint4 load_chunk_edge(const S *src, long long n_ok) {
+  constexpr int EPC = 16 / sizeof(S);
+  if (n_ok >= EPC && (((uintptr_t)src) & 15) == 0) return *(const uint4 *)src; // a whole aligned chunk
+  union { uint4 q; S e[EPC]; } u;
+  const volatile S *vs = src;
+#pragma unroll
+  for (int e = 0; e < EPC; e++) u.e[e] = e < n_ok ? vs[e] : (S)0;
+  return u.q;
+}
+
+
 // f32 -> f32 (T=32, 4 per lane, KF=8): unit t of a plane
 __device__ inline void pack_unit_f32(const PackArgs &a, long long t) {
   int lane = (int)(t & 63);
@@ -313,12 +329,9 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
       if (k < a.K) {
         // (volatile keeps the compiler from merging these edge loads with
         // the interior tile's vector loads)
-        const volatile float *src = (const float *)a.src + (long long)k * a.ks + row0 + r4;
-        long long row = row0 + r4;
-        if (row < a.R) v[it].x = src[0];
-        if (row + 1 < a.R) v[it].y = src[1];
-        if (row + 2 < a.R) v[it].z = src[2];
-        if (row + 3 < a.R) v[it].w = src[3];
+        const float *src = (const float *)a.src + (long long)k * a.ks + row0 + r4;
+        const uint4 q = load_chunk_edge<float>(src, (long long)a.R - (row0 + r4));
+        v[it] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w));
       }
     }
   }
@@ -1176,19 +1189,6 @@ __device__ __attribute__((always_inline)) inline void digits_unit(const PackArgs
 
 // one 16-byte chunk of source values (EPC of them) into LDS as raw 32-bit
 // words; (S)word gives the value back
-// 16 bytes (16 / sizeof(S) values) of an edge chunk, zero past n_ok values;
-// volatile keeps the compiler from merging these scalar loads with the
-// interior tiles' vector loads
-template <class S>
-__device__ __attribute__((always_inline)) inline uint4 load_chunk_edge(const S *src, long long n_ok) {
-  constexpr int EPC = 16 / sizeof(S);
-  union { uint4 q; S e[EPC]; } u;
-  const volatile S *vs = src;
-#pragma unroll
-  for (int e = 0; e < EPC; e++) u.e[e] = e < n_ok ? vs[e] : (S)0;
-  return u.q;
-}
-
 // the values of a 16-byte chunk as 32-bit words; (S)word gives each back
 template <class S>
 __device__ __attribute__((always_inline)) inline void unpack_chunk(uint4 q, uint32_t *v) {
