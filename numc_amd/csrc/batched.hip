@@ -4,7 +4,7 @@
 // No numc counterpart: numc's ROADMAP.md lists numc_bmm as the batched GEMM
 // client (numc_bmm in matmul.cpp runs on this); numc_dot's batched loop
 // (src/reduction/dot.c:409-421) calls the single-problem kernel per batch and
-// runs on this too.  Here the host builds, per call:
+// runs on this too.  For a new set of arrays the host builds:
 //   * a pack descriptor per operand (all packed by one launch),
 //   * a problem table (packed operand addresses, output view),
 //   * a work list of (problem, tile row, tile col, tile class) items ordered
@@ -13,7 +13,9 @@
 //     and numc_matmul_batched_tune picks that threshold by timing the batch,
 // and captures fill-counter -> batched pack -> persistent GEMM into a graph.
 // The graph, its tables and its workspace are cached on the context and
-// replayed while the same arrays (same addresses, shapes, strides) come back.
+// replayed while the same arrays (same addresses, shapes, strides) come back;
+// such a call stages host-resident operands, compares the key and launches
+// the graph, with no planning.
 #include "gemm.h"
 #include "internal.h"
 
@@ -119,7 +121,6 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   };
   const int thresh = numc_batch_threshold(family);
   std::vector<Prob> P(count);
-  long long packed = 0;
   for (size_t i = 0; i < count; i++) {
     Prob &p = P[i];
     p.M = (int)a[i]->shape[0];
@@ -142,62 +143,10 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     if (out[i]->st->residency == RES_HOST && numc_view_span(out[i]) > out[i]->size * eout)
       HIP_TRY(hipMemcpyAsync(dc, out[i]->data, numc_view_span(out[i]), hipMemcpyHostToDevice, ctx->stream), "H2D");
     p.C = dc;
-    const BatchChoice ch = numc_batch_choice(family, p.M, p.N, p.K, thresh);
-    p.cls = ch.cls;
-    p.pred = ch.ms;
-    numc_batch_class_shape(family, p.cls, &p.bm, &p.bn, &p.kc);
-    const long long kp = cdivll(p.K > 0 ? p.K : 1, p.kc) * p.kc;
-    p.nKF = (int)(kp / kf);
-    p.nRTa = (int)(cdivll(p.M, p.bm) * p.bm / 32);
-    p.nRTb = (int)(cdivll(p.N, p.bn) * p.bn / 32);
-    p.pla = (long long)p.nRTa * p.nKF * 1024;
-    p.plb = (long long)p.nRTb * p.nKF * 1024;
-    p.offA = packed;
-    packed += p.pla * planes;
-    p.offB = packed;
-    packed += p.plb * planes;
   }
 
-  // ---- tables: work items longest-first ----
-  std::vector<size_t> order(count);
-  std::iota(order.begin(), order.end(), 0);
-  // longest predicted tile first (a problem's time spread over its tiles)
-  auto tile_time = [&](size_t i) {
-    const Prob &p = P[i];
-    const double tiles = (double)cdivll(p.M, p.bm ? p.bm : 1) * cdivll(p.N, p.bn ? p.bn : 1);
-    return p.pred / (tiles > 0 ? tiles : 1.0);
-  };
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tile_time(x) > tile_time(y); });
-  std::vector<int4> items;
-  std::vector<PackArgs> descs;
-  std::vector<long long> prefix(1, 0);
-  std::vector<BatchProblem> probs(count);
-  for (size_t i = 0; i < count; i++) {
-    const Prob &p = P[i];
-    if (!p.C) continue;
-    PackArgs da{}, db{};
-    da.src = p.A; da.rs = p.rsa; da.ks = p.csa; da.R = p.M; da.K = p.K;
-    da.nRT = p.nRTa; da.nKF = p.nKF; da.plane_bytes = p.pla;
-    db.src = p.B; db.rs = p.csb; db.ks = p.rsb; db.R = p.N; db.K = p.K;
-    db.nRT = p.nRTb; db.nKF = p.nKF; db.plane_bytes = p.plb;
-    const int ba = numc_batch_pack_blocks(family, da), bb = numc_batch_pack_blocks(family, db);
-    descs.push_back(da);
-    prefix.push_back(prefix.back() + ba);
-    descs.push_back(db);
-    prefix.push_back(prefix.back() + bb);
-  }
-  for (size_t oi = 0; oi < count; oi++) {
-    const size_t i = order[oi];
-    const Prob &p = P[i];
-    if (!p.C) continue;
-    const int nbm = (int)cdivll(p.M, p.bm), nbn = (int)cdivll(p.N, p.bn);
-    for (int x = 0; x < nbm; x++)
-      for (int y = 0; y < nbn; y++) items.push_back(make_int4((int)i, x, y, p.cls));
-  }
-  const int ndesc = (int)descs.size();
-  const int nitems = (int)items.size();
-
-  // ---- cache key: everything the graph bakes in ----
+  // ---- cache key: everything the graph bakes in.  A call with the same
+  // arrays as the cached graph replays it and skips the planning below ----
   std::vector<uint64_t> key;
   key.reserve(4 + count * 12);
   key.push_back((uint64_t)family);
@@ -214,77 +163,147 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   if (!ctx->batch_cache) ctx->batch_cache = new BatchCache();
   BatchCache *cache = (BatchCache *)ctx->batch_cache;
 
-  if (nitems > 0 && key != cache->key) {
-    // device layout: descs | prefix | probs | items | counter | packed operands
-    size_t o_desc = 0;
-    size_t o_pref = align_up(o_desc + descs.size() * sizeof(PackArgs), 256);
-    size_t o_prob = align_up(o_pref + prefix.size() * sizeof(long long), 256);
-    size_t o_item = align_up(o_prob + probs.size() * sizeof(BatchProblem), 256);
-    size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
-    size_t o_pack = align_up(o_cnt + 256, 256);
-    size_t total = o_pack + (size_t)packed + 256;
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    cache->release();
-    HIP_TRY(hipMalloc(&cache->dev, total), "hipMalloc(batched workspace)");
-    cache->dev_bytes = total;
-    unsigned char *base = (unsigned char *)cache->dev;
-    // fill in device addresses now that the workspace exists
-    size_t di = 0;
+  bool have_graph = cache->exec && key == cache->key;
+  if (!have_graph) {
+    // ---- plan: tile class and packed layout per problem ----
+    long long packed = 0;
+    for (size_t i = 0; i < count; i++) {
+      Prob &p = P[i];
+      if (!p.C) continue;
+      const BatchChoice ch = numc_batch_choice(family, p.M, p.N, p.K, thresh);
+      p.cls = ch.cls;
+      p.pred = ch.ms;
+      numc_batch_class_shape(family, p.cls, &p.bm, &p.bn, &p.kc);
+      const long long kp = cdivll(p.K > 0 ? p.K : 1, p.kc) * p.kc;
+      p.nKF = (int)(kp / kf);
+      p.nRTa = (int)(cdivll(p.M, p.bm) * p.bm / 32);
+      p.nRTb = (int)(cdivll(p.N, p.bn) * p.bn / 32);
+      p.pla = (long long)p.nRTa * p.nKF * 1024;
+      p.plb = (long long)p.nRTb * p.nKF * 1024;
+      p.offA = packed;
+      packed += p.pla * planes;
+      p.offB = packed;
+      packed += p.plb * planes;
+    }
+    // ---- tables: work items longest-first ----
+    std::vector<size_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    // longest predicted tile first (a problem's time spread over its tiles)
+    auto tile_time = [&](size_t i) {
+      const Prob &p = P[i];
+      const double tiles = (double)cdivll(p.M, p.bm ? p.bm : 1) * cdivll(p.N, p.bn ? p.bn : 1);
+      return p.pred / (tiles > 0 ? tiles : 1.0);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tile_time(x) > tile_time(y); });
+    std::vector<int4> items;
+    std::vector<PackArgs> descs;
+    std::vector<long long> prefix(1, 0);
+    std::vector<BatchProblem> probs(count);
     for (size_t i = 0; i < count; i++) {
       const Prob &p = P[i];
-      BatchProblem &bp = probs[i];
-      memset(&bp, 0, sizeof bp);
       if (!p.C) continue;
-      descs[di].dst = base + o_pack + p.offA;
-      descs[di + 1].dst = base + o_pack + p.offB;
-      di += 2;
-      bp.A = base + o_pack + p.offA;
-      bp.B = base + o_pack + p.offB;
-      bp.a_plane = p.pla;
-      bp.b_plane = p.plb;
-      bp.nKF = p.nKF;
-      bp.nKC = (int)(cdivll(p.K > 0 ? p.K : 1, p.kc));
-      bp.M = p.M;
-      bp.N = p.N;
-      bp.C = p.C;
-      bp.rso = p.rso;
-      bp.cso = p.cso;
-      bp.out_bytes = (int)eout;
+      PackArgs da{}, db{};
+      da.src = p.A; da.rs = p.rsa; da.ks = p.csa; da.R = p.M; da.K = p.K;
+      da.nRT = p.nRTa; da.nKF = p.nKF; da.plane_bytes = p.pla;
+      db.src = p.B; db.rs = p.csb; db.ks = p.rsb; db.R = p.N; db.K = p.K;
+      db.nRT = p.nRTb; db.nKF = p.nKF; db.plane_bytes = p.plb;
+      const int ba = numc_batch_pack_blocks(family, da), bb = numc_batch_pack_blocks(family, db);
+      descs.push_back(da);
+      prefix.push_back(prefix.back() + ba);
+      descs.push_back(db);
+      prefix.push_back(prefix.back() + bb);
     }
-    HIP_TRY(hipMemcpyAsync(base + o_desc, descs.data(), descs.size() * sizeof(PackArgs), hipMemcpyHostToDevice, ctx->stream), "H2D");
-    HIP_TRY(hipMemcpyAsync(base + o_pref, prefix.data(), prefix.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream), "H2D");
-    HIP_TRY(hipMemcpyAsync(base + o_prob, probs.data(), probs.size() * sizeof(BatchProblem), hipMemcpyHostToDevice, ctx->stream), "H2D");
-    HIP_TRY(hipMemcpyAsync(base + o_item, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream), "H2D");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (size_t oi = 0; oi < count; oi++) {
+      const size_t i = order[oi];
+      const Prob &p = P[i];
+      if (!p.C) continue;
+      const int nbm = (int)cdivll(p.M, p.bm), nbn = (int)cdivll(p.N, p.bn);
+      for (int x = 0; x < nbm; x++)
+        for (int y = 0; y < nbn; y++) items.push_back(make_int4((int)i, x, y, p.cls));
+    }
+    const int ndesc = (int)descs.size();
+    const int nitems = (int)items.size();
 
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-    const int grid = std::max(1, std::min(nitems, 2 * cus));
-    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    int lerr = numc_batch_launch(family, (const PackArgs *)(base + o_desc), (const long long *)(base + o_pref), ndesc,
-                                 prefix.back(), (const BatchProblem *)(base + o_prob), (const int4 *)(base + o_item),
-                                 nitems, (int *)(base + o_cnt), grid, ctx->stream);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    if (lerr) {
-      if (graph) (void)hipGraphDestroy(graph);
+    if (nitems > 0) {
+      // owner[b]: the operand (descriptor) that pack block b works on
+      std::vector<int> owner((size_t)prefix.back());
+      for (int d = 0; d < ndesc; d++)
+        std::fill(owner.begin() + prefix[d], owner.begin() + prefix[d + 1], d);
+      // device layout: descs | prefix | owner | probs | items | counter | packed operands
+      size_t o_desc = 0;
+      size_t o_pref = align_up(o_desc + descs.size() * sizeof(PackArgs), 256);
+      size_t o_own = align_up(o_pref + prefix.size() * sizeof(long long), 256);
+      size_t o_prob = align_up(o_own + owner.size() * sizeof(int), 256);
+      size_t o_item = align_up(o_prob + probs.size() * sizeof(BatchProblem), 256);
+      size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
+      size_t o_pack = align_up(o_cnt + 256, 256);
+      size_t total = o_pack + (size_t)packed + 256;
+      HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
       cache->release();
-      return lerr;
+      HIP_TRY(hipMalloc(&cache->dev, total), "hipMalloc(batched workspace)");
+      cache->dev_bytes = total;
+      unsigned char *base = (unsigned char *)cache->dev;
+      // fill in device addresses now that the workspace exists
+      size_t di = 0;
+      for (size_t i = 0; i < count; i++) {
+        const Prob &p = P[i];
+        BatchProblem &bp = probs[i];
+        memset(&bp, 0, sizeof bp);
+        if (!p.C) continue;
+        descs[di].dst = base + o_pack + p.offA;
+        descs[di + 1].dst = base + o_pack + p.offB;
+        di += 2;
+        bp.A = base + o_pack + p.offA;
+        bp.B = base + o_pack + p.offB;
+        bp.a_plane = p.pla;
+        bp.b_plane = p.plb;
+        bp.nKF = p.nKF;
+        bp.nKC = (int)(cdivll(p.K > 0 ? p.K : 1, p.kc));
+        bp.M = p.M;
+        bp.N = p.N;
+        bp.C = p.C;
+        bp.rso = p.rso;
+        bp.cso = p.cso;
+        bp.out_bytes = (int)eout;
+      }
+      HIP_TRY(hipMemcpyAsync(base + o_desc, descs.data(), descs.size() * sizeof(PackArgs), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipMemcpyAsync(base + o_pref, prefix.data(), prefix.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      if (!owner.empty())
+        HIP_TRY(hipMemcpyAsync(base + o_own, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipMemcpyAsync(base + o_prob, probs.data(), probs.size() * sizeof(BatchProblem), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipMemcpyAsync(base + o_item, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+
+      int cus = 256;
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+      const int grid = std::max(1, std::min(nitems, 2 * cus));
+      HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      int lerr = numc_batch_launch(family, (const PackArgs *)(base + o_desc), (const long long *)(base + o_pref),
+                                   (const int *)(base + o_own), prefix.back(), (const BatchProblem *)(base + o_prob),
+                                   (const int4 *)(base + o_item), nitems, (int *)(base + o_cnt), grid, ctx->stream);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      if (lerr) {
+        if (graph) (void)hipGraphDestroy(graph);
+        cache->release();
+        return lerr;
+      }
+      if (e != hipSuccess) {
+        cache->release();
+        return numc_hip_fail(e, "hipStreamEndCapture");
+      }
+      cache->graph = graph;
+      e = hipGraphInstantiate(&cache->exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        cache->release();
+        return numc_hip_fail(e, "hipGraphInstantiate");
+      }
+      cache->key = key;
+      have_graph = true;
     }
-    if (e != hipSuccess) {
-      cache->release();
-      return numc_hip_fail(e, "hipStreamEndCapture");
-    }
-    cache->graph = graph;
-    e = hipGraphInstantiate(&cache->exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      cache->release();
-      return numc_hip_fail(e, "hipGraphInstantiate");
-    }
-    cache->key = key;
   }
 
-  if (nitems > 0) {
+  if (have_graph) {
     numc_prof_begin(ctx, PROF_BATCHED, family);
     HIP_TRY(hipGraphLaunch(cache->exec, ctx->stream), "hipGraphLaunch");
     numc_prof_end(ctx);

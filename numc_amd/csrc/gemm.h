@@ -69,9 +69,9 @@ BatchChoice numc_batch_choice(int family, int M, int N, int K, int thresh);
 int numc_batch_threshold(int family);
 void numc_batch_set_threshold(int family, int t);
 int numc_batch_pack_blocks(int family, PackArgs &pk); // sets pk.mode
-int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
-                      const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
-                      hipStream_t st);
+int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, const int *owner,
+                      long long units, const BatchProblem *probs, const int4 *items, int nitems, int *counter,
+                      int grid, hipStream_t st);
 
 const GemmConfig *numc_gemm_config(int family, int idx);
 // configuration for a strided batch of equal M x N slices

@@ -1563,21 +1563,19 @@ __global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
 
 // ---------------------------------------------------------------------------
 // batched pack: every operand of every problem in one launch.  Operand i owns
-// blocks [prefix[i], prefix[i+1]); its `mode` (set on the host) says which
-// tile walk it takes: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 one
-// thread per 16-byte lane slot (any other strides).
+// blocks [prefix[i], prefix[i+1]) and owner[b] names the operand of block b
+// (one load, where a search of prefix would take a chain of dependent ones);
+// its `mode` (set on the host) says which tile walk it takes: 0 k-contiguous
+// tiles, 1 row-contiguous tiles, 2 one thread per 16-byte lane slot (any
+// other strides), 3/4 the byte tiles of int8 operands.
 // ---------------------------------------------------------------------------
 template <int FAM>
-__global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, const long long *prefix, int ndesc) {
+__global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, const long long *prefix,
+                                                      const int *owner, long long units) {
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   const long long b = blockIdx.x;
-  if (b >= prefix[ndesc]) return;
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
+  if (b >= units) return;
+  const int lo = owner[b];
   const PackArgs a = descs[lo];
   const long long u = b - prefix[lo];
   constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : 1;
@@ -1975,15 +1973,15 @@ int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p) {
   return 0;
 }
 
-int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, int ndesc, long long units,
-                      const BatchProblem *probs, const int4 *items, int nitems, int *counter, int grid,
-                      hipStream_t st) {
+int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, const int *owner,
+                      long long units, const BatchProblem *probs, const int4 *items, int nitems, int *counter,
+                      int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, 1LL, 0u);
   const unsigned nb = (unsigned)units; // blocks (numc_batch_pack_blocks summed)
   if (nb) {
-    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_F32>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
-    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_SPLIT_BF16>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
-    else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, ndesc);
+    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_F32>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
+    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_SPLIT_BF16>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
+    else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
   }
   if (family == NUMC_FAMILY_F32)
     hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
