@@ -78,14 +78,23 @@ __global__ void k_zero_strided(unsigned char *c, int M, int N, long long rso, lo
 
 template <class S> __device__ inline S ld(const void *p, long long i) { return ((const S *)p)[i]; }
 
+// a vector load from an address aligned only to its elements: global vector
+// loads on gfx950 take any byte alignment (a chunk that straddles two cache
+// lines costs one more line), and the compiler emits the same
+// global_load_dwordx4 for this copy as for an aligned load
+template <class V> __device__ __attribute__((always_inline)) inline V ldu(const void *p) {
+  V v;
+  __builtin_memcpy(&v, p, sizeof(V));
+  return v;
+}
+
 // 16 bytes (16 / sizeof(S) values) of a chunk of an edge tile, zero past
-// n_ok values: one vector load when the chunk is whole and aligned, else
-// element loads (volatile keeps the compiler from merging them with the
+// n_ok values: one vector load when the chunk is whole, else element loads (volatile keeps the compiler from merging them with the
 // interior tiles' vector loads)
 template <class S>
 __device__ __attribute__((always_inline)) inline uint4 load_chunk_edge(const S *src, long long n_ok) {
   constexpr int EPC = 16 / sizeof(S);
-  if (n_ok >= EPC && (((uintptr_t)src) & 15) == 0) return *(const uint4 *)src; // a whole aligned chunk
+  if (n_ok >= EPC) return ldu<uint4>(src); // a whole chunk
   union { uint4 q; S e[EPC]; } u;
   const volatile S *vs = src;
 #pragma unroll
@@ -213,12 +222,12 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
   // 32 rows x 128 floats = 1024 float4, 4 per thread
   float4 v[4];
   const float *base = (const float *)a.src + row0 * a.rs + k0;
-  if (row0 + 32 <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.rs * 4)) & 15) == 0) {
-    // interior tile, 16-byte aligned rows: every load in flight at once
+  if (row0 + 32 <= a.R && k0 + TK <= a.K) {
+    // interior tile: every load in flight at once
 #pragma unroll
     for (int it = 0; it < 4; it++) {
       const int idx = tid + it * 256;
-      v[it] = *(const float4 *)(base + (idx >> 5) * a.rs + (idx & 31) * 4);
+      v[it] = ldu<float4>(base + (idx >> 5) * a.rs + (idx & 31) * 4);
     }
   } else {
 #pragma unroll
@@ -227,17 +236,10 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
       int r = idx >> 5, c4 = (idx & 31) * 4;
       long long row = row0 + r;
       v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (row < a.R) {
-        const float *src = (const float *)a.src + row * a.rs;
-        int k = k0 + c4;
-        if (k + 3 < a.K && ((((uintptr_t)(src + k)) & 15) == 0)) {
-          v[it] = *(const float4 *)(src + k);
-        } else {
-          if (k < a.K) v[it].x = src[k];
-          if (k + 1 < a.K) v[it].y = src[k + 1];
-          if (k + 2 < a.K) v[it].z = src[k + 2];
-          if (k + 3 < a.K) v[it].w = src[k + 3];
-        }
+      const int k = k0 + c4;
+      if (row < a.R && k < a.K) {
+        const uint4 q = load_chunk_edge<float>((const float *)a.src + row * a.rs + k, (long long)a.K - k);
+        v[it] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w));
       }
     }
   }
@@ -312,12 +314,12 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
   // 32 k x 128 rows = 1024 float4 along rows, 4 per thread
   float4 v[4];
   const float *base = (const float *)a.src + (long long)k0 * a.ks + row0;
-  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.ks * 4)) & 15) == 0) {
-    // interior tile, 16-byte aligned: every load in flight at once
+  if (row0 + TR <= a.R && k0 + TK <= a.K) {
+    // interior tile: every load in flight at once
 #pragma unroll
     for (int it = 0; it < 4; it++) {
       const int idx = tid + it * 256;
-      v[it] = *(const float4 *)(base + (long long)(idx >> 5) * a.ks + (idx & 31) * 4);
+      v[it] = ldu<float4>(base + (long long)(idx >> 5) * a.ks + (idx & 31) * 4);
     }
   } else {
 #pragma unroll
@@ -1215,12 +1217,12 @@ __device__ __attribute__((always_inline)) inline void digits_tile_k(const PackAr
   const int k0 = kt * TK;
   const S *base = (const S *)a.src + row0 * a.rs + k0;
   uint4 q[NL];
-  if (row0 + 32 <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.rs * sizeof(S))) & 15) == 0) {
-    // interior tile, 16-byte aligned rows: every load in flight at once
+  if (row0 + 32 <= a.R && k0 + TK <= a.K) {
+    // interior tile: every load in flight at once
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256;
-      q[it] = *(const uint4 *)(base + (idx / CPR) * a.rs + (idx % CPR) * EPC);
+      q[it] = ldu<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * EPC);
     }
   } else {
 #pragma unroll
@@ -1273,12 +1275,12 @@ __device__ __attribute__((always_inline)) inline void digits_tile_rows(const Pac
   const int k0 = kt * TK;
   const S *base = (const S *)a.src + (long long)k0 * a.ks + row0;
   uint4 q[NL];
-  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.ks * sizeof(S))) & 15) == 0) {
-    // interior tile, 16-byte aligned: every load in flight at once
+  if (row0 + TR <= a.R && k0 + TK <= a.K) {
+    // interior tile: every load in flight at once
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256;
-      q[it] = *(const uint4 *)(base + (long long)(idx / CPK) * a.ks + (idx % CPK) * EPC);
+      q[it] = ldu<uint4>(base + (long long)(idx / CPK) * a.ks + (idx % CPK) * EPC);
     }
   } else {
 #pragma unroll
@@ -1339,12 +1341,12 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArg
   const int tid = threadIdx.x;
   const unsigned char *base = (const unsigned char *)a.src + rt * 32 * a.rs + (long long)kb * KB;
   uint4 v[NL];
-  if (rt * 32 + 32 <= a.R && kb * KB + KB <= a.K && ((((uintptr_t)base) | (uintptr_t)a.rs) & 15) == 0) {
-    // interior tile, 16-byte aligned rows: every load in flight at once
+  if (rt * 32 + 32 <= a.R && kb * KB + KB <= a.K) {
+    // interior tile: every load in flight at once
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256;
-      v[it] = *(const uint4 *)(base + (idx / CPR) * a.rs + (idx % CPR) * 16);
+      v[it] = ldu<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * 16);
     }
   } else {
 #pragma unroll
@@ -1404,10 +1406,10 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_rows(const Pack
   uint32_t w[4][4]; // [k offset][dword of 4 rows]
   uint4 v[4];
   const unsigned char *base = (const unsigned char *)a.src + (long long)k0 * a.ks + row0;
-  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)a.ks) & 15) == 0) {
-    // interior tile, 16-byte aligned: all four loads in flight at once
+  if (row0 + TR <= a.R && k0 + TK <= a.K) {
+    // interior tile: all four loads in flight at once
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = *(const uint4 *)(base + (long long)(kq * 4 + q) * a.ks + rg * 16);
+    for (int q = 0; q < 4; q++) v[q] = ldu<uint4>(base + (long long)(kq * 4 + q) * a.ks + rg * 16);
   } else {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -1496,10 +1498,10 @@ __device__ __attribute__((always_inline)) inline void raw16_tile_rows(const Pack
   const uint16_t *src = (const uint16_t *)a.src;
   const uint16_t *base = src + (long long)k0 * a.ks + row0;
   uint4 v[4];
-  if (row0 + TR <= a.R && k0 + TK <= a.K && ((((uintptr_t)base) | (uintptr_t)(a.ks * 2)) & 15) == 0) {
-    // interior tile, 16-byte aligned: all four loads in flight at once
+  if (row0 + TR <= a.R && k0 + TK <= a.K) {
+    // interior tile: all four loads in flight at once
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = *(const uint4 *)(base + (long long)(kq * 4 + q) * a.ks + rg * 8);
+    for (int q = 0; q < 4; q++) v[q] = ldu<uint4>(base + (long long)(kq * 4 + q) * a.ks + rg * 8);
   } else {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
