@@ -291,6 +291,56 @@ def wsum(x):
     return int(np.sum(w * np.arange(1, w.size + 1, dtype=np.uint64)))
 
 
+def bench_vendor(steps, warmup, seed):
+    """The vendor GEMMs that torch dispatches to (hipBLASLt, rocBLAS) on the shapes timed here, so the
+    roofline fractions can be read against what the vendor library itself reaches on this GPU: bf16,
+    fp16 and fp32 8192^3 (torch.matmul), int8 -> int32 4096^3 (torch._int_mm) and fp8 e4m3 8192^3
+    with per-tensor scales (torch._scaled_mm).  Not part of the library; torch is only the caller."""
+    import torch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    res = {}
+
+    def timed(name, fn, flop):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        res[name] = {"ms": round(ms, 5), "tflops": round(flop / ms * 1e-9, 2)}
+        log("vendor: %s %.4f ms, %.1f TFLOP/s" % (name, ms, flop / ms * 1e-9))
+
+    n = 8192
+    for name, dt in (("bf16_8192", torch.bfloat16), ("f16_8192", torch.float16), ("f32_8192", torch.float32)):
+        a = torch.randn(n, n, device=dev, dtype=dt, generator=g)
+        b = torch.randn(n, n, device=dev, dtype=dt, generator=g)
+        timed(name, lambda: torch.matmul(a, b), 2.0 * n ** 3)
+        del a, b
+    m = 4096
+    try:
+        a = torch.randint(-128, 128, (m, m), device=dev, dtype=torch.int8, generator=g)
+        b = torch.randint(-128, 128, (m, m), device=dev, dtype=torch.int8, generator=g)
+        timed("int8_4096", lambda: torch._int_mm(a, b), 2.0 * m ** 3)
+    except (RuntimeError, AttributeError) as e:
+        res["int8_4096"] = {"unsupported": str(e).splitlines()[0][:200]}
+    try:
+        f8 = torch.float8_e4m3fn
+        a = (torch.randn(n, n, device=dev, generator=g) * 0.5).to(f8)
+        b = (torch.randn(n, n, device=dev, generator=g) * 0.5).to(f8).t().contiguous().t()  # column-major
+        one = torch.ones((), device=dev)
+        timed("fp8_8192", lambda: torch._scaled_mm(a, b, one, one, out_dtype=torch.float32), 2.0 * n ** 3)
+    except (RuntimeError, AttributeError, TypeError) as e:
+        res["fp8_8192"] = {"unsupported": str(e).splitlines()[0][:200]}
+    print(json.dumps({"vendor": "torch %s (hipBLASLt/rocBLAS)" % torch.__version__, "steps": steps,
+                      "warmup": warmup, "results": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,12 +359,18 @@ def main():
                          "bf16/f16 8192^3, numc_linear, numc_bmm)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the reference copy kernel over this many MiB (HBM counter calibration)")
+    ap.add_argument("--vendor", action="store_true",
+                    help="only time torch.matmul / torch._int_mm / torch._scaled_mm (hipBLASLt, rocBLAS) at the "
+                         "same shapes, as a calibration of what the vendor library reaches on this GPU")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (one context drives "
               "one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.vendor:
+        bench_vendor(args.steps, args.warmup, args.seed)
+        return
     import numc_amd as nc
     if args.copy_probe:
         ctx = nc.Ctx()
