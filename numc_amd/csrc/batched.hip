@@ -276,7 +276,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
 
       int cus = 256;
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-      const int grid = std::max(1, std::min(nitems, 2 * cus));
+      const int grid = std::max(1, std::min(nitems, numc_batch_occupancy(family) * cus));
       HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       int lerr = numc_batch_launch(family, (const PackArgs *)(base + o_desc), (const long long *)(base + o_pref),
                                    (const int *)(base + o_own), prefix.back(), (const BatchProblem *)(base + o_prob),

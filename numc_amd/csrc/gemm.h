@@ -61,6 +61,7 @@ struct BatchProblem {
 };
 
 int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc);
+int numc_batch_occupancy(int family);
 struct BatchChoice {
   int cls;  // 0 = 128x128 tiles, 1 = 64x64
   float ms; // predicted time of the whole problem

@@ -930,12 +930,14 @@ __global__ void k_gemm_i64(const T *a, const T *b, T *c, int M, int K, int N, lo
 // problems get small tiles and the queue is ordered longest-first.
 // ===========================================================================
 
-template <class MF, int MODE, int EPI, int BM1, int BN1, int KC1, int NS1, int BM2, int BN2, int KC2, int NS2>
-__global__ void __launch_bounds__(256, 2) k_gemm_batched(const BatchProblem *probs, const int4 *items, int nitems,
+template <class MF, int MODE, int EPI, int BM1, int BN1, int KC1, int NS1, int BM2, int BN2, int KC2, int NS2,
+          int OCC>
+__global__ void __launch_bounds__(256, OCC) k_gemm_batched(const BatchProblem *probs, const int4 *items, int nitems,
                                                           int *counter) {
   using S1 = TileShape<MF, BM1, BN1, KC1, 2, 2, MODE, NS1>;
   using S2 = TileShape<MF, BM2, BN2, KC2, 2, 2, MODE, NS2>;
   constexpr int LDSV = S1::LDS_VEC > S2::LDS_VEC ? S1::LDS_VEC : S2::LDS_VEC;
+  static_assert(OCC * LDSV * 16 <= 160 * 1024, "OCC blocks must fit one CU's LDS");
   __shared__ u32x4 lds[LDSV];
   __shared__ int s_item;
   for (;;) {
@@ -970,10 +972,15 @@ __global__ void __launch_bounds__(256, 2) k_gemm_batched(const BatchProblem *pro
 
 // tile classes of the batched kernel per family: {BM, BN, KC}
 static const int kBatchClass[3][2][3] = {
-    {{128, 128, 32}, {64, 64, 32}},   // f32
+    {{128, 128, 16}, {64, 64, 32}},   // f32
     {{128, 128, 32}, {64, 64, 32}},   // split-BF16
     {{128, 128, 64}, {64, 64, 64}},   // int8 (one digit plane)
 };
+
+// resident blocks per CU of the batched kernel (its grid is this many per CU)
+int numc_batch_occupancy(int family) {
+  return family == NUMC_FAMILY_I8 ? 4 : family == NUMC_FAMILY_F32 ? 3 : 2;
+}
 
 int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc) {
   int f = family == NUMC_FAMILY_F32 ? 0 : family == NUMC_FAMILY_SPLIT_BF16 ? 1 : family == NUMC_FAMILY_I8 ? 2 : -1;
@@ -1986,11 +1993,11 @@ int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix
     else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
   }
   if (family == NUMC_FAMILY_F32)
-    hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+    hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 16, 3, 64, 64, 32, 3, 3>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   else if (family == NUMC_FAMILY_SPLIT_BF16)
-    hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+    hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4, 2>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   else
-    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 4, 64, 64, 64, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 2, 64, 64, 64, 4, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "batched launch");
   return 0;
