@@ -20,6 +20,7 @@
 #include "internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -221,6 +222,9 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       for (int x = 0; x < nbm; x++)
         for (int y = 0; y < nbn; y++) items.push_back(make_int4((int)i, x, y, p.cls));
     }
+    if (items.size() > (size_t)INT_MAX || prefix.back() > (long long)INT_MAX)
+      return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched: too many tiles for one launch (%zu work items, %lld pack blocks)",
+                            items.size(), prefix.back());
     const int ndesc = (int)descs.size();
     const int nitems = (int)items.size();
 
