@@ -1578,15 +1578,9 @@ __global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
 // tiles, 1 row-contiguous tiles, 2 one thread per 16-byte lane slot (any
 // other strides), 3/4 the byte tiles of int8 operands.
 // ---------------------------------------------------------------------------
+// block u of operand a, by the walk a.mode names
 template <int FAM>
-__global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, const long long *prefix,
-                                                      const int *owner, long long units) {
-  __shared__ uint32_t tile[PACK_TILE_WORDS];
-  const long long b = blockIdx.x;
-  if (b >= units) return;
-  const int lo = owner[b];
-  const PackArgs a = descs[lo];
-  const long long u = b - prefix[lo];
+__device__ __attribute__((always_inline)) inline void pack_walk(const PackArgs &a, long long u, uint32_t *tile) {
   constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : 1;
   if constexpr (FAM == NUMC_FAMILY_I8) {
     if (a.mode == 3) {
@@ -1612,6 +1606,26 @@ __global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, con
       else pack_unit_i8(a, t);
     }
   }
+}
+
+template <int FAM>
+__global__ void __launch_bounds__(256) k_pack_batched(const PackArgs *descs, const long long *prefix,
+                                                      const int *owner, long long units) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  const long long b = blockIdx.x;
+  if (b >= units) return;
+  const int lo = owner[b];
+  const PackArgs a = descs[lo];
+  pack_walk<FAM>(a, b - prefix[lo], tile);
+}
+
+// both operands of one problem in one launch (one launch gap less per call):
+// blocks [0, na) pack A, the others B
+template <int FAM>
+__global__ void __launch_bounds__(256) k_pack_pair(const PackArgs a, const PackArgs b, unsigned na) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  if (blockIdx.x < na) pack_walk<FAM>(a, blockIdx.x, tile);
+  else pack_walk<FAM>(b, blockIdx.x - na, tile);
 }
 
 // blocks the batched pack gives one operand, and the walk it takes
@@ -1799,23 +1813,38 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     launch_rowmax((const float *)p.B, p.csb, p.rsb, p.N, p.K, bitsB, st);
     hipLaunchKernelGGL(k_exps, dim3(blocks_for(p.M + p.N)), dim3(256), 0, st, bitsA, expA, p.M + p.N);
   }
-  if (family == NUMC_FAMILY_I8)
+  // one pack launch for both operands where the batched walks cover the
+  // family.  A single int8 digit plane needs no plane masks (they only let
+  // the GEMM skip empty plane products), so that case sets none.
+  const bool pair = nz == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
+                                (family == NUMC_FAMILY_I8 && planes == 1 && p.in_dt == NUMC_DTYPE_INT8));
+  const bool masks = family == NUMC_FAMILY_I8 && !pair;
+  if (masks)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * NUMC_FLAG_INTS)), dim3(256), 0, st, (uint32_t *)flags,
                        (long long)(2 * NUMC_FLAG_INTS), 0u);
 
   // A: rows M, element (i, k) at A[i*rsa + k*csa]
   pk.src = p.A; pk.rs = p.rsa; pk.ks = p.csa; pk.R = p.M; pk.K = p.K;
   pk.nRT = nRTa; pk.nKF = nKF; pk.dst = Ap; pk.plane_bytes = pla;
-  pk.row_exp = expA; pk.flags = flags;
+  pk.row_exp = expA; pk.flags = masks ? flags : nullptr;
   pk.src_zs = p.a_bs * (long long)ein; pk.dst_zs = pla * planes;
-  numc_prof_begin(ctx, PROF_PACK, family);
-  launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
   // B^T: rows N, element (j, k) at B[k*rsb + j*csb]
-  pk.src = p.B; pk.rs = p.csb; pk.ks = p.rsb; pk.R = p.N; pk.K = p.K;
-  pk.nRT = nRTb; pk.nKF = nKF; pk.dst = Bp; pk.plane_bytes = plb;
-  pk.row_exp = expB; pk.flags = flags + NUMC_FLAG_INTS;
-  pk.src_zs = p.b_bs * (long long)ein; pk.dst_zs = plb * planes;
-  launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
+  PackArgs pb = pk;
+  pb.src = p.B; pb.rs = p.csb; pb.ks = p.rsb; pb.R = p.N; pb.K = p.K;
+  pb.nRT = nRTb; pb.nKF = nKF; pb.dst = Bp; pb.plane_bytes = plb;
+  pb.row_exp = expB; pb.flags = masks ? flags + NUMC_FLAG_INTS : nullptr;
+  pb.src_zs = p.b_bs * (long long)ein; pb.dst_zs = plb * planes;
+  numc_prof_begin(ctx, PROF_PACK, family);
+  if (pair) {
+    const long long na = numc_batch_pack_blocks(family, pk), nb = numc_batch_pack_blocks(family, pb);
+    const dim3 grid((unsigned)(na + nb));
+    if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F32>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+    else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_SPLIT_BF16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+    else hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_I8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+  } else {
+    launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
+    launch_pack(family, p.in_dt, planes, pb, (unsigned)nz, st);
+  }
   numc_prof_end(ctx);
 
   GemmArgs g{};
@@ -1849,7 +1878,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   const int passes = family == NUMC_FAMILY_I8 ? planes : 1;
   for (int s = 0; s < passes; s++) {
     g.shift_pass = s;
-    g.plane_flags = family == NUMC_FAMILY_I8 ? flags : nullptr;
+    g.plane_flags = masks ? flags : nullptr;
     numc_prof_begin(ctx, PROF_GEMM, family);
     (bias && c.launch_bias ? c.launch_bias : c.launch)(g, grid, st);
     numc_prof_end(ctx);
