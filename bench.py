@@ -353,7 +353,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--only", default=None, help="time one method: split_bf16, f32 or fp8")
+    ap.add_argument("--only", default=None,
+                    help="time one method: split_bf16, f32 or fp8; or 'batched' for the uneven batch alone")
     ap.add_argument("--suite", action="store_true",
                     help="time the other configurations (f32 512^3, f64 2048^3, int32 2048^3, int8 4096^3, "
                          "bf16/f16 8192^3, numc_linear, numc_bmm)")
@@ -387,6 +388,10 @@ def main():
         return
     if args.suite:
         bench_suite(nc, args.steps, args.warmup, args.seed)
+        return
+    if args.only == "batched":
+        log("bench: batched uneven GEMM, %d problems" % args.batch)
+        print(json.dumps(bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2))), flush=True)
         return
     n = args.n
     flops = 2.0 * n * n * n

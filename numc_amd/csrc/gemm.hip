@@ -1161,7 +1161,11 @@ static int family_kf(int family) {
   }
 }
 
-static int digit_planes(NumcDType out) {
+// Digit planes of an integer GEMM: one per output byte (the int32 MFMA sum is
+// exact mod 2^32), except that an int8 input is its own single signed digit,
+// whatever the output width.
+static int digit_planes(NumcDType in, NumcDType out) {
+  if (in == NUMC_DTYPE_INT8) return 1;
   size_t s = numc_dtype_size(out);
   return s >= 4 ? 4 : (int)s;
 }
@@ -1757,7 +1761,7 @@ size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *au
   long long pa, pb;
   plane_geometry(c, p.M, p.K, kf, c.cfg.bm, &nRT, &nKF, &pa);
   plane_geometry(c, p.N, p.K, kf, c.cfg.bn, &nRT, &nKF, &pb);
-  int planes = p.family == NUMC_FAMILY_SPLIT_BF16 ? 2 : (p.family == NUMC_FAMILY_I8 ? digit_planes(p.out_dt) : 1);
+  int planes = p.family == NUMC_FAMILY_SPLIT_BF16 ? 2 : (p.family == NUMC_FAMILY_I8 ? digit_planes(p.in_dt, p.out_dt) : 1);
   size_t raw = (p.family == NUMC_FAMILY_F64) ? 8 : 4;
   size_t partial = c.cfg.split_k > 1 ? (size_t)c.cfg.split_k * p.M * p.N * raw : 0;
   const size_t nz = p.batch > 1 ? (size_t)p.batch : 1;
@@ -1770,7 +1774,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   hipStream_t st = ctx->stream;
   const int family = p.family;
   const int kf = family_kf(family);
-  const int planes = family == NUMC_FAMILY_SPLIT_BF16 ? 2 : (family == NUMC_FAMILY_I8 ? digit_planes(p.out_dt) : 1);
+  const int planes = family == NUMC_FAMILY_SPLIT_BF16 ? 2 : (family == NUMC_FAMILY_I8 ? digit_planes(p.in_dt, p.out_dt) : 1);
 
   if (p.bias && (family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_FP8))
     return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: a bias needs the plain method (family %d)", family);
