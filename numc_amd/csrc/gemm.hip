@@ -629,17 +629,22 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave / WGN, wn = wave % WGN;
 
-  // plane products: product q uses plane (pa0 + q) of A and (pb0 - q) of B
+  // plane products: product q uses plane (pa0 + q) of A and (pb0 - q) of B.
+  // Integer digit passes: a launch per pass adds (sum_s << 8s) into C, or,
+  // with g.passes > 1, the tile runs every pass itself, from the highest
+  // down, shifting its accumulators 8 bits left between passes (Horner), and
+  // its one store writes sum_s (sum_s << 8s) mod 2^32.
+  constexpr bool FUSE = std::is_same<MF, MfI8>::value;
+  const int npass = FUSE && g.passes > 1 ? g.passes : 1;
   int nprod = g.nprod, pa0 = g.pa[0], pb0 = g.pb[0];
+  int ma = 0, mb = 0;
   if (g.plane_flags) {
-    int ma, mb;
     plane_masks(g.plane_flags, &ma, &mb);
-    nprod = int_products(g.shift_pass, ma, mb, &pa0, &pb0);
-    if (nprod == 0 && g.shift_pass > 0) return; // this digit pass adds nothing
+    if (npass == 1) {
+      nprod = int_products(g.shift_pass, ma, mb, &pa0, &pb0);
+      if (nprod == 0 && g.shift_pass > 0) return; // this digit pass adds nothing
+    }
   }
-  const int V = nprod * g.nKC;
-  const int c0 = (int)((long long)slice * V / g.ksplit);
-  const int c1 = (int)((long long)(slice + 1) * V / g.ksplit);
 
   // wave-uniform byte offsets of the fragments this wave brings into LDS
   long long uoffA[LA], uoffB[LB];
@@ -742,63 +747,82 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   constexpr int NKS = KFC / FR;
   constexpr int NRD = P * (TM + TN) * FR;                                 // LDS reads per k step
   constexpr int NMF = TM * TN * (MODE == MODE_SPLIT3 ? 3 : 1) * MF::NMMA; // MFMAs per k step
-  if (c0 < c1) {
-    const int clast = c1 - 1;
-    int stage = 0;
-    auto body = [&](auto b0, int c, auto is_last) __attribute__((always_inline)) {
-      constexpr int B0 = decltype(b0)::value;
-      const int next = stage == NS - 1 ? 0 : stage + 1;
+  for (int pass = npass - 1; pass >= 0; pass--) {
+    if constexpr (FUSE) {
+      if (npass > 1) {
+        if (pass < npass - 1) {
 #pragma unroll
-      for (int ks = 0; ks < NKS; ks++) {
-        const int buf = (B0 + ks) & 1;
-        if (ks + 1 < NKS) {
-          fload(stage, (ks + 1) * FR, buf ^ 1);
-          mfmas(buf);
-          Interleave<0, NRD, NMF>::apply();
-        } else if constexpr (!decltype(is_last)::value) {
-          __builtin_amdgcn_sched_barrier(0);
-          WAIT_VMCNT(LPW * (NS - 2));
-          lds_barrier();
-          issue(min(c + NS, clast), stage);
-          fload(next, 0, buf ^ 1);
-          mfmas(buf);
-          Interleave<LPW, NRD, NMF>::apply();
-        } else {
-          mfmas(buf);
+          for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+              for (int v = 0; v < MF::NACC; v++) acc[i][j][v] = (int)((uint32_t)acc[i][j][v] << 8);
+          __syncthreads(); // every wave is done reading the last pass's stages
         }
-      }
-      stage = next;
-    };
-    using I0 = std::integral_constant<int, 0>;
-    using I1 = std::integral_constant<int, 1>;
-    using More = std::false_type;
-    using Last = std::true_type;
-#pragma unroll
-    for (int u = 0; u < NS; u++) issue(min(c0 + u, clast), u);
-    WAIT_VMCNT(LPW * (NS - 1));
-    lds_barrier();
-    fload(0, 0, 0);
-    int c = c0;
-    if constexpr (NKS % 2 == 0) {
-      for (; c < clast; c++) body(I0(), c, More());
-      body(I0(), c, Last());
-    } else {
-      // the register buffer alternates per chunk: two chunks per iteration
-      for (; c + 1 < clast; c += 2) {
-        body(I0(), c, More());
-        body(I1(), c + 1, More());
-      }
-      if (c < clast) {
-        body(I0(), c, More());
-        body(I1(), c + 1, Last());
-      } else {
-        body(I0(), c, Last());
+        if (g.plane_flags) nprod = int_products(pass, ma, mb, &pa0, &pb0);
       }
     }
-    // the clamped copies past the last chunk may still be in flight: let
-    // them land before the stages are reused or the workgroup ends
-    WAIT_VMCNT(0);
-  }
+    const int V = nprod * g.nKC;
+    const int c0 = (int)((long long)slice * V / g.ksplit);
+    const int c1 = (int)((long long)(slice + 1) * V / g.ksplit);
+    if (c0 < c1) {
+      const int clast = c1 - 1;
+      int stage = 0;
+      auto body = [&](auto b0, int c, auto is_last) __attribute__((always_inline)) {
+        constexpr int B0 = decltype(b0)::value;
+        const int next = stage == NS - 1 ? 0 : stage + 1;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+          const int buf = (B0 + ks) & 1;
+          if (ks + 1 < NKS) {
+            fload(stage, (ks + 1) * FR, buf ^ 1);
+            mfmas(buf);
+            Interleave<0, NRD, NMF>::apply();
+          } else if constexpr (!decltype(is_last)::value) {
+            __builtin_amdgcn_sched_barrier(0);
+            WAIT_VMCNT(LPW * (NS - 2));
+            lds_barrier();
+            issue(min(c + NS, clast), stage);
+            fload(next, 0, buf ^ 1);
+            mfmas(buf);
+            Interleave<LPW, NRD, NMF>::apply();
+          } else {
+            mfmas(buf);
+          }
+        }
+        stage = next;
+      };
+      using I0 = std::integral_constant<int, 0>;
+      using I1 = std::integral_constant<int, 1>;
+      using More = std::false_type;
+      using Last = std::true_type;
+#pragma unroll
+      for (int u = 0; u < NS; u++) issue(min(c0 + u, clast), u);
+      WAIT_VMCNT(LPW * (NS - 1));
+      lds_barrier();
+      fload(0, 0, 0);
+      int c = c0;
+      if constexpr (NKS % 2 == 0) {
+        for (; c < clast; c++) body(I0(), c, More());
+        body(I0(), c, Last());
+      } else {
+        // the register buffer alternates per chunk: two chunks per iteration
+        for (; c + 1 < clast; c += 2) {
+          body(I0(), c, More());
+          body(I1(), c + 1, More());
+        }
+        if (c < clast) {
+          body(I0(), c, More());
+          body(I1(), c + 1, Last());
+        } else {
+          body(I0(), c, Last());
+        }
+      }
+      // the clamped copies past the last chunk may still be in flight: let
+      // them land before the stages are reused or the workgroup ends
+      WAIT_VMCNT(0);
+    }
+  } // digit passes
 
   // epilogue (split-K slices store raw partial sums; their reduction stores)
   const int row0 = bm * BM + wm * WM, col0 = bn * BN + wn * WN;
@@ -1879,38 +1903,38 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.c_zs = p.c_bs * (long long)eout;
   dim3 grid((unsigned)(g.nbm * g.nbn), (unsigned)g.ksplit, (unsigned)nz);
   const bool bias = p.bias != nullptr;
-  const int passes = family == NUMC_FAMILY_I8 ? planes : 1;
-  for (int s = 0; s < passes; s++) {
-    g.shift_pass = s;
-    g.plane_flags = masks ? flags : nullptr;
-    numc_prof_begin(ctx, PROF_GEMM, family);
-    (bias && c.launch_bias ? c.launch_bias : c.launch)(g, grid, st);
-    numc_prof_end(ctx);
-    if (g.ksplit > 1) {
-      unsigned nb = blocks_for((long long)p.M * p.N);
-      switch (family) {
-      case NUMC_FAMILY_F64:
-        if (bias) hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64B>), dim3(nb), dim3(256), 0, st, g);
-        else hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64>), dim3(nb), dim3(256), 0, st, g);
-        break;
-      case NUMC_FAMILY_FP8: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_FP8>), dim3(nb), dim3(256), 0, st, g); break;
-      case NUMC_FAMILY_BF16:
-        if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16B>), dim3(nb), dim3(256), 0, st, g);
-        else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16>), dim3(nb), dim3(256), 0, st, g);
-        break;
-      case NUMC_FAMILY_F16:
-        if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F16B>), dim3(nb), dim3(256), 0, st, g);
-        else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F16>), dim3(nb), dim3(256), 0, st, g);
-        break;
-      case NUMC_FAMILY_I8:
-        if (bias) hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INTB>), dim3(nb), dim3(256), 0, st, g);
-        else hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INT>), dim3(nb), dim3(256), 0, st, g);
-        break;
-      default:
-        if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32B>), dim3(nb), dim3(256), 0, st, g);
-        else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32>), dim3(nb), dim3(256), 0, st, g);
-        break;
-      }
+  // integer digit planes: one launch runs every pass (the tile shifts its
+  // accumulators between passes), so the empty passes cost no launches
+  g.passes = family == NUMC_FAMILY_I8 ? planes : 1;
+  g.shift_pass = 0;
+  g.plane_flags = masks ? flags : nullptr;
+  numc_prof_begin(ctx, PROF_GEMM, family);
+  (bias && c.launch_bias ? c.launch_bias : c.launch)(g, grid, st);
+  numc_prof_end(ctx);
+  if (g.ksplit > 1) {
+    unsigned nb = blocks_for((long long)p.M * p.N);
+    switch (family) {
+    case NUMC_FAMILY_F64:
+      if (bias) hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64B>), dim3(nb), dim3(256), 0, st, g);
+      else hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64>), dim3(nb), dim3(256), 0, st, g);
+      break;
+    case NUMC_FAMILY_FP8: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_FP8>), dim3(nb), dim3(256), 0, st, g); break;
+    case NUMC_FAMILY_BF16:
+      if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16B>), dim3(nb), dim3(256), 0, st, g);
+      else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16>), dim3(nb), dim3(256), 0, st, g);
+      break;
+    case NUMC_FAMILY_F16:
+      if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F16B>), dim3(nb), dim3(256), 0, st, g);
+      else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F16>), dim3(nb), dim3(256), 0, st, g);
+      break;
+    case NUMC_FAMILY_I8:
+      if (bias) hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INTB>), dim3(nb), dim3(256), 0, st, g);
+      else hipLaunchKernelGGL((k_splitk_reduce<int, EPI_INT>), dim3(nb), dim3(256), 0, st, g);
+      break;
+    default:
+      if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32B>), dim3(nb), dim3(256), 0, st, g);
+      else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32>), dim3(nb), dim3(256), 0, st, g);
+      break;
     }
   }
   hipError_t e = hipGetLastError();

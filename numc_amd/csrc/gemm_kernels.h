@@ -153,6 +153,7 @@ struct GemmArgs {
   long long rso, cso;         // output strides in elements
   int out_bytes;              // integer output element size (1, 2, 4)
   int shift_pass;             // integer digit pass s (adds (sum << 8s))
+  int passes;                 // > 1: every digit pass in this launch (int8 MFMA kernels), shift_pass 0
   const int *plane_flags;     // integer: [0] nonzero-plane mask of A, [1] of B (null: use pa/pb)
   const int *row_exp, *col_exp; // FP8 power-of-two scale exponents
   void *partial;              // split-K partial sums [ksplit][M][N] (raw accumulator type)
