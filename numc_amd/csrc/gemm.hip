@@ -181,10 +181,7 @@ __global__ void k_pack_split_bf16(PackArgs a) {
 
 
 // f32 * 2^row_exp[row] -> e4m3  (T=32, 16 per lane, KF=32)
-__global__ void k_pack_fp8(PackArgs a) {
-  zslice(a);
-  long long t = blockIdx.x * 256LL + threadIdx.x;
-  if (t >= (long long)a.nRT * a.nKF * 64) return;
+__device__ inline void pack_unit_fp8(const PackArgs &a, long long t) {
   int lane = (int)(t & 63);
   long long fr = t >> 6;
   int kf = (int)(fr % a.nKF);
@@ -199,6 +196,12 @@ __global__ void k_pack_fp8(PackArgs a) {
     w[e >> 2] |= e4m3_rne(ldexpf(x, ex)) << (8 * (e & 3));
   }
   *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ void k_pack_fp8(PackArgs a) {
+  zslice(a);
+  long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t < (long long)a.nRT * a.nKF * 64) pack_unit_fp8(a, t);
 }
 
 // ---- tiled packs for f32 rows with unit k-stride ------------------------
@@ -1609,7 +1612,7 @@ __global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
 // block u of operand a, by the walk a.mode names
 template <int FAM>
 __device__ __attribute__((always_inline)) inline void pack_walk(const PackArgs &a, long long u, uint32_t *tile) {
-  constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : 1;
+  constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : FAM == NUMC_FAMILY_FP8 ? 2 : 1;
   if constexpr (FAM == NUMC_FAMILY_I8) {
     if (a.mode == 3) {
       bytes_tile_k(a, u, tile);
@@ -1631,6 +1634,7 @@ __device__ __attribute__((always_inline)) inline void pack_walk(const PackArgs &
     if (t < (long long)a.nRT * a.nKF * 64) {
       if constexpr (FAM == NUMC_FAMILY_F32) pack_unit_f32(a, t);
       else if constexpr (FAM == NUMC_FAMILY_SPLIT_BF16) pack_unit_split(a, t);
+      else if constexpr (FAM == NUMC_FAMILY_FP8) pack_unit_fp8(a, t);
       else pack_unit_i8(a, t);
     }
   }
@@ -1845,6 +1849,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // family.  A single int8 digit plane needs no plane masks (they only let
   // the GEMM skip empty plane products), so that case sets none.
   const bool pair = nz == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
+                                family == NUMC_FAMILY_FP8 ||
                                 (family == NUMC_FAMILY_I8 && planes == 1 && p.in_dt == NUMC_DTYPE_INT8));
   const bool masks = family == NUMC_FAMILY_I8 && !pair;
   if (masks)
@@ -1868,6 +1873,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     const dim3 grid((unsigned)(na + nb));
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F32>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_SPLIT_BF16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+    else if (family == NUMC_FAMILY_FP8) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_FP8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_I8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
   } else {
     launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
