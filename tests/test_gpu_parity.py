@@ -249,6 +249,24 @@ def test_widen_int8_to_int32(ctx):
         assert np.array_equal(got, oracle.matmul_int(a, b, odt)), (dt, odt)
 
 
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_I8)))
+def test_widen_int8_configs(ctx, cfg):
+    """int8 -> int32 takes one digit plane (an int8 value is its own signed
+    digit): every configuration, split-K ones included, and a sum that wraps
+    mod 2^32 (all -128 entries, K = 2^17 + 40: K * 2^14 exceeds 2^31)."""
+    rng = np.random.default_rng(620 + cfg)
+    a = rng.integers(-128, 127, (130, 1000), dtype=np.int8, endpoint=True)
+    b = rng.integers(-128, 127, (1000, 70), dtype=np.int8, endpoint=True)
+    got = gemm(ctx, a, b, out_dtype=np.int32, flags=nc.OPT_WIDEN, config=cfg)
+    assert np.array_equal(got, oracle.matmul_int(a, b, np.int32)), cfg
+    K = (1 << 17) + 40
+    a = np.full((33, K), -128, np.int8)
+    b = np.full((K, 17), -128, np.int8)
+    want = np.full((33, 17), np.int64(K) * 16384, np.int64).astype(np.uint32).view(np.int32)
+    assert np.array_equal(oracle.matmul_int(a, b, np.int32), want)
+    assert np.array_equal(gemm(ctx, a, b, out_dtype=np.int32, flags=nc.OPT_WIDEN, config=cfg), want), cfg
+
+
 # ---- views, residency, empty shapes -----------------------------------------------
 
 def test_strided_views(ctx):
