@@ -524,6 +524,8 @@ def main():
                                     "(three bf16 products per FP32 product)" % flops},
         "cpu_baseline": cpu,
         "split_vs_plain_speedup": round(s["tflops"] / p["tflops"], 3),
+        # the target compares the MFMA kernels: split-BF16 kernel TFLOPS / plain FP32 kernel TFLOPS
+        "split_vs_plain_kernel_speedup": round(s["kernel_tflops"] / p["kernel_tflops"], 3),
         "variants": {k: {kk: (float("%.5g" % vv) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }

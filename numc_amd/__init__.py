@@ -27,7 +27,7 @@ def bf16_bits(x):
     u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
     nan = (u & 0x7F800000) == 0x7F800000
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
-    q = ((u >> 16) | np.where((u & 0x7FFFFF) != 0, 0x40, 0)).astype(np.uint16)
+    q = ((u >> 16) | np.where((u & 0x7FFFFF) != 0, np.uint64(0x40), np.uint64(0))).astype(np.uint16)
     return np.where(nan, q, r).astype(np.uint16)
 
 

@@ -34,6 +34,7 @@ struct GemmConfig {
   int T; // rows per MFMA fragment
   GemmLaunchFn launch;
   GemmLaunchFn launch_bias; // same kernel with a per-column bias in the store (float families), or null
+  GemmLaunchFn launch_adir; // same kernel reading A in place (GemmArgs::Araw), or null
 };
 
 struct PackArgs {

@@ -664,6 +664,21 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     uoffB[i] = p * g.b_plane + ((long long)(bn * RB + rt) * g.nKF + kf) * NUMC_FRAG_BYTES;
   }
   const int loff = lane * 16;
+  // A read in place: lane l's 16 bytes of a fragment are row (l & 31) of its
+  // row tile, bytes 32*kf + 16*(l >> 5) of the k-chunk (the packed layout's
+  // order), so the copies land in LDS exactly as the packed fragments do.
+  // Rows past M repeat row M-1: their products reach no stored output.
+  constexpr bool ADIR = (VAR & 2) != 0;
+  static_assert(!ADIR || P == 1, "A is read in place only with one plane");
+  const unsigned char *adir[ADIR ? LA : 1];
+  if constexpr (ADIR) {
+#pragma unroll
+    for (int i = 0; i < LA; i++) {
+      const int f = wave + i * NW, rt = (f / KFC) % RA, kf = f % KFC;
+      const long long row = min((long long)(bm * RA + rt) * 32 + (lane & 31), (long long)g.M - 1);
+      adir[i] = g.Araw + row * g.a_rs + kf * 32 + (lane >> 5) * 16;
+    }
+  }
 
   Acc acc[TM][TN];
 #pragma unroll
@@ -681,10 +696,18 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     const unsigned char *pA = g.A + ko + (long long)(pa0 + q) * g.a_plane + loff;
     const unsigned char *pB = g.B + ko + (long long)(pb0 - q) * g.b_plane + loff;
     u32x4 *base = lds + s * FRC * 64;
+    if constexpr (ADIR) {
+      const long long kb = (long long)kc * KFC * 32; // 32 bytes of each row per k-fragment
 #pragma unroll
-    for (int i = 0; i < LA; i++)
-      __builtin_amdgcn_global_load_lds((global_void_t *)(pA + uoffA[i]),
-                                       (lds_void_t *)(base + (wave + i * NW) * 64), 16, 0, 0);
+      for (int i = 0; i < LA; i++)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(adir[i] + kb),
+                                         (lds_void_t *)(base + (wave + i * NW) * 64), 16, 0, 0);
+    } else {
+#pragma unroll
+      for (int i = 0; i < LA; i++)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(pA + uoffA[i]),
+                                         (lds_void_t *)(base + (wave + i * NW) * 64), 16, 0, 0);
+    }
 #pragma unroll
     for (int i = 0; i < LB; i++)
       __builtin_amdgcn_global_load_lds((global_void_t *)(pB + uoffB[i]),
@@ -1036,16 +1059,24 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
   {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>}
 
+// ... and also the twin that reads A in place (FP32 and int8; the 16-bit
+// kernels consume A fast enough that the gathered copies slowed them by 10 %
+// at 8192^3, more than the pack they save)
+#define CFGD(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                      \
+  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>,                                  \
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 2>}
+
 // (family, tile, k-chunk, waves, split-K, LDS stages); stages x chunk bytes <= 160 KiB
 static const GemmConfig kF32[] = {
-    CFGB(MfF32, 256, 256, 32, 2, 4, EPI_F32, 1, 2),
-    CFGB(MfF32, 256, 256, 16, 2, 2, EPI_F32, 1, 4),
-    CFGB(MfF32, 256, 128, 16, 2, 2, EPI_F32, 1, 4),
-    CFGB(MfF32, 128, 256, 16, 2, 2, EPI_F32, 1, 4),
-    CFGB(MfF32, 128, 128, 32, 2, 2, EPI_F32, 1, 4),
-    CFGB(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 4),
-    CFGB(MfF32, 128, 128, 32, 2, 2, EPI_F32, 4, 4),
-    CFGB(MfF32, 64, 64, 32, 2, 2, EPI_F32, 8, 4),
+    CFGD(MfF32, 256, 256, 32, 2, 4, EPI_F32, 1, 2),
+    CFGD(MfF32, 256, 256, 16, 2, 2, EPI_F32, 1, 4),
+    CFGD(MfF32, 256, 128, 16, 2, 2, EPI_F32, 1, 4),
+    CFGD(MfF32, 128, 256, 16, 2, 2, EPI_F32, 1, 4),
+    CFGD(MfF32, 128, 128, 32, 2, 2, EPI_F32, 1, 4),
+    CFGD(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 4),
+    CFGD(MfF32, 128, 128, 32, 2, 2, EPI_F32, 4, 4),
+    CFGD(MfF32, 64, 64, 32, 2, 2, EPI_F32, 8, 4),
 };
 static const GemmConfig kSplit[] = {
     CFGP(MfBF16, 256, 256, 32, 4, 2, MODE_SPLIT3, EPI_F32, 1, 2, 1),
@@ -1076,12 +1107,12 @@ static const GemmConfig kF64[] = {
     CFGB(MfF64, 128, 128, 32, 2, 2, EPI_F64, 1, 2),
 };
 static const GemmConfig kI8[] = {
-    CFGB(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 4),
-    CFGB(MfI8, 256, 128, 64, 2, 2, EPI_INT, 1, 4),
-    CFGB(MfI8, 64, 64, 128, 2, 2, EPI_INT, 1, 4),
-    CFGB(MfI8, 128, 128, 64, 2, 2, EPI_INT, 4, 4),
-    CFGB(MfI8, 256, 256, 64, 4, 2, EPI_INT, 1, 4),
-    CFGB(MfI8, 256, 256, 128, 4, 2, EPI_INT, 1, 2),
+    CFGD(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 4),
+    CFGD(MfI8, 256, 128, 64, 2, 2, EPI_INT, 1, 4),
+    CFGD(MfI8, 64, 64, 128, 2, 2, EPI_INT, 1, 4),
+    CFGD(MfI8, 128, 128, 64, 2, 2, EPI_INT, 4, 4),
+    CFGD(MfI8, 256, 256, 64, 4, 2, EPI_INT, 1, 4),
+    CFGD(MfI8, 256, 256, 128, 4, 2, EPI_INT, 1, 2),
 };
 
 // bfloat16 / float16 arrays: one MFMA product (fragments twice as dense in k
@@ -1852,6 +1883,13 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
                                 family == NUMC_FAMILY_FP8 ||
                                 (family == NUMC_FAMILY_I8 && planes == 1 && p.in_dt == NUMC_DTYPE_INT8));
   const bool masks = family == NUMC_FAMILY_I8 && !pair;
+  // A read in place by the GEMM (no pack of A): FP32 and single-plane int8,
+  // A's rows k-contiguous and 16-byte aligned, K a multiple of the k-chunk
+  // (every copy stays inside its row)
+  const bool adir = nz == 1 && c.launch_adir && !p.bias && !masks && p.csa == 1 &&
+                    (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8) &&
+                    ((uintptr_t)p.A & 15) == 0 && ((long long)p.rsa * (long long)ein) % 16 == 0 &&
+                    p.K % c.cfg.kc == 0;
   if (masks)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * NUMC_FLAG_INTS)), dim3(256), 0, st, (uint32_t *)flags,
                        (long long)(2 * NUMC_FLAG_INTS), 0u);
@@ -1868,7 +1906,9 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   pb.row_exp = expB; pb.flags = masks ? flags + NUMC_FLAG_INTS : nullptr;
   pb.src_zs = p.b_bs * (long long)ein; pb.dst_zs = plb * planes;
   numc_prof_begin(ctx, PROF_PACK, family);
-  if (pair) {
+  if (adir) {
+    launch_pack(family, p.in_dt, planes, pb, 1u, st);
+  } else if (pair) {
     const long long na = numc_batch_pack_blocks(family, pk), nb = numc_batch_pack_blocks(family, pb);
     const dim3 grid((unsigned)(na + nb));
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F32>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
@@ -1907,6 +1947,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.a_zs = pla * planes;
   g.b_zs = plb * planes;
   g.c_zs = p.c_bs * (long long)eout;
+  g.Araw = (const unsigned char *)p.A;
+  g.a_rs = (long long)p.rsa * (long long)ein;
   dim3 grid((unsigned)(g.nbm * g.nbn), (unsigned)g.ksplit, (unsigned)nz);
   const bool bias = p.bias != nullptr;
   // integer digit planes: one launch runs every pass (the tile shifts its
@@ -1915,7 +1957,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.shift_pass = 0;
   g.plane_flags = masks ? flags : nullptr;
   numc_prof_begin(ctx, PROF_GEMM, family);
-  (bias && c.launch_bias ? c.launch_bias : c.launch)(g, grid, st);
+  (bias && c.launch_bias ? c.launch_bias : adir ? c.launch_adir : c.launch)(g, grid, st);
   numc_prof_end(ctx);
   if (g.ksplit > 1) {
     unsigned nb = blocks_for((long long)p.M * p.N);

@@ -162,6 +162,8 @@ struct GemmArgs {
   const void *bias;           // optional per-column bias (output dtype), added at the store
   long long bias_s;           // bias stride in elements
   long long a_zs, b_zs, c_zs; // strided batch: bytes between slices (blockIdx.z) of A, B (packed) and C
+  const unsigned char *Araw;  // A read in place (kernels compiled with VAR & 2): k-contiguous rows,
+  long long a_rs;             // a_rs bytes apart, 16-byte aligned, K a multiple of the k-chunk
 };
 
 // Nonzero-digit-plane masks of the integer packs: each operand's mask is

@@ -606,3 +606,49 @@ def test_batched_fallback_dtypes_and_empty(ctx):
     assert cw_err(Os[0].numpy(), *ref[0]) <= 42 * U64
     assert Os[1].numpy().shape == (0, 3)
     assert np.all(Os[2].numpy() == 0)
+
+
+# ---- A read in place (no pack of A) --------------------------------------------
+
+@pytest.mark.parametrize("name", ["float32", "bfloat16", "float16", "int8"])
+@pytest.mark.parametrize("offset", [0, 1], ids=["aligned", "offset"])
+def test_a_read_in_place(ctx, name, offset):
+    """float32 and int8: an A whose rows are k-contiguous and 16-byte aligned,
+    with K a multiple of the configuration's k-chunk, is copied into LDS by the
+    GEMM itself (no pack of A): every configuration, M ragged against every
+    tile, A a view whose rows sit 320 elements apart.  offset 1 starts the view
+    one element in, so its rows are unaligned and A is packed: both paths give
+    the same bits.  bfloat16 and float16 always pack A; they run the same cases.
+    Floats hold small integers, so every sum is exact in the FP32 accumulator
+    and the one rounding to the output dtype is known (round to nearest even)."""
+    rng = np.random.default_rng(700 + offset)
+    M, K, N, W = 300, 256, 200, 320
+    if name == "int8":
+        fam, dt = nc.FAMILY_I8, nc.INT8
+        base = rng.integers(-128, 127, (M, W), dtype=np.int8, endpoint=True)
+        b = rng.integers(-128, 127, (K, N), dtype=np.int8, endpoint=True)
+        a = base[:, offset:offset + K]
+        want = oracle.matmul_int(np.ascontiguousarray(a), b)
+        A, B = ctx.from_numpy(base), ctx.from_numpy(b)
+    else:
+        fam = {"float32": nc.FAMILY_F32, "bfloat16": nc.FAMILY_BF16, "float16": nc.FAMILY_F16}[name]
+        dt = {"float32": nc.FLOAT32, "bfloat16": nc.BFLOAT16, "float16": nc.FLOAT16}[name]
+        base_i = rng.integers(-4, 4, (M, W), endpoint=True)
+        b_i = rng.integers(-4, 4, (K, N), endpoint=True)
+        exact = (base_i[:, offset:offset + K] @ b_i).astype(np.float32)  # |sum| <= 4096: exact
+        if name == "float32":
+            A, B = ctx.from_numpy(base_i.astype(np.float32)), ctx.from_numpy(b_i.astype(np.float32))
+            want = exact
+        elif name == "float16":
+            A, B = ctx.from_numpy(base_i.astype(np.float16)), ctx.from_numpy(b_i.astype(np.float16))
+            want = exact.astype(np.float16)
+        else:
+            A = ctx.from_numpy(nc.bf16_bits(base_i.astype(np.float32)), dtype=nc.BFLOAT16)
+            B = ctx.from_numpy(nc.bf16_bits(b_i.astype(np.float32)), dtype=nc.BFLOAT16)
+            want = nc.bf16_bits(exact)
+    Av = A.slice(1, offset, offset + K)
+    for cfg in range(nc.config_count(fam)):
+        O = ctx.array((M, N), dt, zeros=True)
+        nc.check(nc.matmul(Av, B, O, nc.METHOD_AUTO, cfg), "matmul A in place")
+        got = O.numpy()
+        assert np.array_equal(got, want), (name, offset, cfg)

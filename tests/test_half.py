@@ -77,6 +77,19 @@ def test_oracle_bf16_rounding_closed_forms():
     assert np.isnan(oracle.bf16_to_f32(b[6:]))[0]
 
 
+def test_python_bf16_bits_matches_oracle():
+    """numc_amd.bf16_bits (the Python helper) rounds as the oracle does, on
+    random bits of every class (2-D input), and bf16_value inverts it"""
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 2 ** 32, (64, 257), dtype=np.uint64).astype(np.uint32).view(np.float32)
+    got = nc.bf16_bits(x)
+    want = oracle.split_bf16(x.ravel())[0].reshape(x.shape)
+    fin = np.isfinite(x)
+    assert got.shape == x.shape and np.array_equal(got[fin], want[fin])
+    assert np.all(np.isnan(nc.bf16_value(got[np.isnan(x)])))
+    assert np.array_equal(nc.bf16_bits(nc.bf16_value(got[fin])), got[fin])
+
+
 def test_validation(ctx):
     a, b = ctx.array((2, 3), nc.BFLOAT16), ctx.array((3, 4), nc.FLOAT16)
     o = ctx.array((2, 4), nc.BFLOAT16)
