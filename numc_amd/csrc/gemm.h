@@ -35,6 +35,7 @@ struct GemmConfig {
   GemmLaunchFn launch;
   GemmLaunchFn launch_bias; // same kernel with a per-column bias in the store (float families), or null
   GemmLaunchFn launch_adir; // same kernel reading A in place (GemmArgs::Araw), or null
+  GemmLaunchFn launch_adir_bias; // reading A in place, with the bias store, or null
 };
 
 struct PackArgs {

@@ -1065,7 +1065,8 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
 #define CFGD(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                      \
   {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>,                                  \
-   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 2>}
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 2>,                                      \
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 2>}
 
 // (family, tile, k-chunk, waves, split-K, LDS stages); stages x chunk bytes <= 160 KiB
 static const GemmConfig kF32[] = {
@@ -1886,7 +1887,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // A read in place by the GEMM (no pack of A): FP32 and single-plane int8,
   // A's rows k-contiguous and 16-byte aligned, K a multiple of the k-chunk
   // (every copy stays inside its row)
-  const bool adir = nz == 1 && c.launch_adir && !p.bias && !masks && p.csa == 1 &&
+  const bool adir = nz == 1 && (p.bias ? c.launch_adir_bias : c.launch_adir) && !masks && p.csa == 1 &&
                     (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8) &&
                     ((uintptr_t)p.A & 15) == 0 && ((long long)p.rsa * (long long)ein) % 16 == 0 &&
                     p.K % c.cfg.kc == 0;
@@ -1957,7 +1958,10 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.shift_pass = 0;
   g.plane_flags = masks ? flags : nullptr;
   numc_prof_begin(ctx, PROF_GEMM, family);
-  (bias && c.launch_bias ? c.launch_bias : adir ? c.launch_adir : c.launch)(g, grid, st);
+  GemmLaunchFn fn = c.launch;
+  if (adir) fn = bias ? c.launch_adir_bias : c.launch_adir;
+  else if (bias && c.launch_bias) fn = c.launch_bias;
+  fn(g, grid, st);
   numc_prof_end(ctx);
   if (g.ksplit > 1) {
     unsigned nb = blocks_for((long long)p.M * p.N);

@@ -221,14 +221,15 @@ def test_bmm_transposed_operand_and_empty(ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dt", [np.float32, np.float64, np.int32, np.uint8], ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("dt", [np.float32, np.float64, np.int32, np.uint8, np.int8], ids=lambda d: np.dtype(d).name)
 def test_linear_every_config(ctx, dt):
     """the biased store of every kernel configuration of the dtype's family,
-    split-K ones included (their reduction adds the bias)"""
+    split-K ones included (their reduction adds the bias); K = 512 with an
+    aligned x is the case where float32 and int8 kernels read x in place"""
     fam = {np.float32: nc.FAMILY_F32, np.float64: nc.FAMILY_F64}.get(dt, nc.FAMILY_I8)
     rng = np.random.default_rng(zlib.crc32(np.dtype(dt).str.encode()))
     for cfg in range(nc.config_count(fam)):
-        for M, K, N in [(130, 257, 70), (1, 700, 33)]:
+        for M, K, N in [(130, 257, 70), (1, 700, 33), (100, 512, 90)]:
             x, w, bias = _gen(rng, dt, (M, K)), _gen(rng, dt, (K, N)), _gen(rng, dt, (N,))
             O = ctx.array((M, N), DT[dt], zeros=True)
             nc.check(nc.linear(ctx.from_numpy(x), ctx.from_numpy(w), ctx.from_numpy(bias), O,
