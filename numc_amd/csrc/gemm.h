@@ -34,8 +34,9 @@ struct GemmConfig {
   int T; // rows per MFMA fragment
   GemmLaunchFn launch;
   GemmLaunchFn launch_bias; // same kernel with a per-column bias in the store (float families), or null
-  GemmLaunchFn launch_adir; // same kernel reading A in place (GemmArgs::Araw), or null
-  GemmLaunchFn launch_adir_bias; // reading A in place, with the bias store, or null
+  // same kernel reading operands in place (GemmArgs::Araw / Braw), or null:
+  // [A, B, A and B][plain store, bias store]
+  GemmLaunchFn inplace[3][2];
 };
 
 struct PackArgs {

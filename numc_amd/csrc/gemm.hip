@@ -668,15 +668,25 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   // row tile, bytes 32*kf + 16*(l >> 5) of the k-chunk (the packed layout's
   // order), so the copies land in LDS exactly as the packed fragments do.
   // Rows past M repeat row M-1: their products reach no stored output.
-  constexpr bool ADIR = (VAR & 2) != 0;
-  static_assert(!ADIR || P == 1, "A is read in place only with one plane");
+  // B in place (VAR & 4) likewise, with the rows of B^T.
+  constexpr bool ADIR = (VAR & 2) != 0, BDIR = (VAR & 4) != 0;
+  static_assert(!(ADIR || BDIR) || P == 1, "operands are read in place only with one plane");
   const unsigned char *adir[ADIR ? LA : 1];
+  const unsigned char *bdir[BDIR ? LB : 1];
   if constexpr (ADIR) {
 #pragma unroll
     for (int i = 0; i < LA; i++) {
       const int f = wave + i * NW, rt = (f / KFC) % RA, kf = f % KFC;
       const long long row = min((long long)(bm * RA + rt) * 32 + (lane & 31), (long long)g.M - 1);
       adir[i] = g.Araw + row * g.a_rs + kf * 32 + (lane >> 5) * 16;
+    }
+  }
+  if constexpr (BDIR) {
+#pragma unroll
+    for (int i = 0; i < LB; i++) {
+      const int f = wave + i * NW, rt = (f / KFC) % RB, kf = f % KFC;
+      const long long row = min((long long)(bn * RB + rt) * 32 + (lane & 31), (long long)g.N - 1);
+      bdir[i] = g.Braw + row * g.b_rs + kf * 32 + (lane >> 5) * 16;
     }
   }
 
@@ -696,8 +706,8 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     const unsigned char *pA = g.A + ko + (long long)(pa0 + q) * g.a_plane + loff;
     const unsigned char *pB = g.B + ko + (long long)(pb0 - q) * g.b_plane + loff;
     u32x4 *base = lds + s * FRC * 64;
+    const long long kb = (long long)kc * KFC * 32; // in place: 32 bytes of each row per k-fragment
     if constexpr (ADIR) {
-      const long long kb = (long long)kc * KFC * 32; // 32 bytes of each row per k-fragment
 #pragma unroll
       for (int i = 0; i < LA; i++)
         __builtin_amdgcn_global_load_lds((global_void_t *)(adir[i] + kb),
@@ -708,10 +718,17 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
         __builtin_amdgcn_global_load_lds((global_void_t *)(pA + uoffA[i]),
                                          (lds_void_t *)(base + (wave + i * NW) * 64), 16, 0, 0);
     }
+    if constexpr (BDIR) {
 #pragma unroll
-    for (int i = 0; i < LB; i++)
-      __builtin_amdgcn_global_load_lds((global_void_t *)(pB + uoffB[i]),
-                                       (lds_void_t *)(base + (A_FR + wave + i * NW) * 64), 16, 0, 0);
+      for (int i = 0; i < LB; i++)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(bdir[i] + kb),
+                                         (lds_void_t *)(base + (A_FR + wave + i * NW) * 64), 16, 0, 0);
+    } else {
+#pragma unroll
+      for (int i = 0; i < LB; i++)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(pB + uoffB[i]),
+                                         (lds_void_t *)(base + (A_FR + wave + i * NW) * 64), 16, 0, 0);
+    }
   };
   // fragments of one k step, double buffered in registers
   u32x4 af[2][P][TM][FR], bfr[2][P][TN][FR];
@@ -1059,19 +1076,30 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
   {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>}
 
-// ... and also the twin that reads A in place (FP32 and int8; the 16-bit
-// kernels consume A fast enough that the gathered copies slowed them by 10 %
-// at 8192^3, more than the pack they save)
+// ... and also the twins that read A, B or both in place (FP32 and int8; the
+// 16-bit kernels consume A fast enough that the gathered copies slowed them
+// by 10 % at 8192^3, more than the pack they save)
+#define CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, V)                                              \
+  {&launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, V>,                            \
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, V>}
 #define CFGD(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                      \
   {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>,                                  \
-   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 2>,                                      \
-   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 2>}
+   {CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 2), CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 4),             \
+    CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 6)}}
+// A in place only: int8 (its kernel slowed more with both operands gathered
+// than the B pack costs: 4096^3 with B^T rows contiguous 0.0995 ms against
+// 0.0951 ms with B packed) and an FP32 configuration whose B-in-place twins
+// would spill
+#define CFGDA(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                     \
+  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
+   &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>,                                  \
+   {CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 2), {nullptr, nullptr}, {nullptr, nullptr}}}
 
 // (family, tile, k-chunk, waves, split-K, LDS stages); stages x chunk bytes <= 160 KiB
 static const GemmConfig kF32[] = {
     CFGD(MfF32, 256, 256, 32, 2, 4, EPI_F32, 1, 2),
-    CFGD(MfF32, 256, 256, 16, 2, 2, EPI_F32, 1, 4),
+    CFGDA(MfF32, 256, 256, 16, 2, 2, EPI_F32, 1, 4),
     CFGD(MfF32, 256, 128, 16, 2, 2, EPI_F32, 1, 4),
     CFGD(MfF32, 128, 256, 16, 2, 2, EPI_F32, 1, 4),
     CFGD(MfF32, 128, 128, 32, 2, 2, EPI_F32, 1, 4),
@@ -1108,12 +1136,12 @@ static const GemmConfig kF64[] = {
     CFGB(MfF64, 128, 128, 32, 2, 2, EPI_F64, 1, 2),
 };
 static const GemmConfig kI8[] = {
-    CFGD(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 4),
-    CFGD(MfI8, 256, 128, 64, 2, 2, EPI_INT, 1, 4),
-    CFGD(MfI8, 64, 64, 128, 2, 2, EPI_INT, 1, 4),
-    CFGD(MfI8, 128, 128, 64, 2, 2, EPI_INT, 4, 4),
-    CFGD(MfI8, 256, 256, 64, 4, 2, EPI_INT, 1, 4),
-    CFGD(MfI8, 256, 256, 128, 4, 2, EPI_INT, 1, 2),
+    CFGDA(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 4),
+    CFGDA(MfI8, 256, 128, 64, 2, 2, EPI_INT, 1, 4),
+    CFGDA(MfI8, 64, 64, 128, 2, 2, EPI_INT, 1, 4),
+    CFGDA(MfI8, 128, 128, 64, 2, 2, EPI_INT, 4, 4),
+    CFGDA(MfI8, 256, 256, 64, 4, 2, EPI_INT, 1, 4),
+    CFGDA(MfI8, 256, 256, 128, 4, 2, EPI_INT, 1, 2),
 };
 
 // bfloat16 / float16 arrays: one MFMA product (fragments twice as dense in k
@@ -1887,10 +1915,14 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // A read in place by the GEMM (no pack of A): FP32 and single-plane int8,
   // A's rows k-contiguous and 16-byte aligned, K a multiple of the k-chunk
   // (every copy stays inside its row)
-  const bool adir = nz == 1 && (p.bias ? c.launch_adir_bias : c.launch_adir) && !masks && p.csa == 1 &&
-                    (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8) &&
-                    ((uintptr_t)p.A & 15) == 0 && ((long long)p.rsa * (long long)ein) % 16 == 0 &&
-                    p.K % c.cfg.kc == 0;
+  const bool inplace_ok = nz == 1 && c.inplace[0][0] && !masks && p.K % c.cfg.kc == 0 &&
+                          (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8);
+  const bool adir = inplace_ok && p.csa == 1 && ((uintptr_t)p.A & 15) == 0 &&
+                    ((long long)p.rsa * (long long)ein) % 16 == 0;
+  // B likewise when the rows of B^T are k-contiguous (B stored transposed):
+  // FP32 only
+  const bool bdir = inplace_ok && c.inplace[1][0] && p.rsb == 1 && ((uintptr_t)p.B & 15) == 0 &&
+                    ((long long)p.csb * (long long)ein) % 16 == 0;
   if (masks)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * NUMC_FLAG_INTS)), dim3(256), 0, st, (uint32_t *)flags,
                        (long long)(2 * NUMC_FLAG_INTS), 0u);
@@ -1907,8 +1939,12 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   pb.row_exp = expB; pb.flags = masks ? flags + NUMC_FLAG_INTS : nullptr;
   pb.src_zs = p.b_bs * (long long)ein; pb.dst_zs = plb * planes;
   numc_prof_begin(ctx, PROF_PACK, family);
-  if (adir) {
+  if (adir && bdir) {
+    // nothing to pack
+  } else if (adir) {
     launch_pack(family, p.in_dt, planes, pb, 1u, st);
+  } else if (bdir) {
+    launch_pack(family, p.in_dt, planes, pk, 1u, st);
   } else if (pair) {
     const long long na = numc_batch_pack_blocks(family, pk), nb = numc_batch_pack_blocks(family, pb);
     const dim3 grid((unsigned)(na + nb));
@@ -1950,6 +1986,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.c_zs = p.c_bs * (long long)eout;
   g.Araw = (const unsigned char *)p.A;
   g.a_rs = (long long)p.rsa * (long long)ein;
+  g.Braw = (const unsigned char *)p.B;
+  g.b_rs = (long long)p.csb * (long long)ein;
   dim3 grid((unsigned)(g.nbm * g.nbn), (unsigned)g.ksplit, (unsigned)nz);
   const bool bias = p.bias != nullptr;
   // integer digit planes: one launch runs every pass (the tile shifts its
@@ -1959,7 +1997,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.plane_flags = masks ? flags : nullptr;
   numc_prof_begin(ctx, PROF_GEMM, family);
   GemmLaunchFn fn = c.launch;
-  if (adir) fn = bias ? c.launch_adir_bias : c.launch_adir;
+  if (adir || bdir) fn = c.inplace[adir && bdir ? 2 : (bdir ? 1 : 0)][bias ? 1 : 0];
   else if (bias && c.launch_bias) fn = c.launch_bias;
   fn(g, grid, st);
   numc_prof_end(ctx);

@@ -164,6 +164,8 @@ struct GemmArgs {
   long long a_zs, b_zs, c_zs; // strided batch: bytes between slices (blockIdx.z) of A, B (packed) and C
   const unsigned char *Araw;  // A read in place (kernels compiled with VAR & 2): k-contiguous rows,
   long long a_rs;             // a_rs bytes apart, 16-byte aligned, K a multiple of the k-chunk
+  const unsigned char *Braw;  // B read in place (VAR & 4): the same for the rows of B^T
+  long long b_rs;
 };
 
 // Nonzero-digit-plane masks of the integer packs: each operand's mask is

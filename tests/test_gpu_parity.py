@@ -652,3 +652,35 @@ def test_a_read_in_place(ctx, name, offset):
         nc.check(nc.matmul(Av, B, O, nc.METHOD_AUTO, cfg), "matmul A in place")
         got = O.numpy()
         assert np.array_equal(got, want), (name, offset, cfg)
+
+
+@pytest.mark.parametrize("name", ["float32", "int8"])
+@pytest.mark.parametrize("a_off,b_off", [(0, 0), (1, 0), (0, 1), (1, 1)],
+                         ids=["both-aligned", "a-offset", "b-offset", "both-offset"])
+def test_b_read_in_place(ctx, name, a_off, b_off):
+    """B stored transposed (a torch-style (N, K) weight passed as its transposed
+    view): the rows of B^T are k-contiguous, and when they are 16-byte aligned
+    with K a multiple of the k-chunk the float32 GEMM copies them in place
+    too, with or without A in place (int8 reads only A in place; it runs the
+    same cases).  An offset of one element makes that operand unaligned, so it
+    is packed.  Every configuration; exact sums."""
+    rng = np.random.default_rng(800 + 2 * a_off + b_off)
+    M, K, N, W = 300, 256, 200, 320
+    if name == "int8":
+        fam, dt = nc.FAMILY_I8, nc.INT8
+        base_a = rng.integers(-128, 127, (M, W), dtype=np.int8, endpoint=True)
+        base_w = rng.integers(-128, 127, (N, W), dtype=np.int8, endpoint=True)
+        a, w = base_a[:, a_off:a_off + K], base_w[:, b_off:b_off + K]
+        want = oracle.matmul_int(np.ascontiguousarray(a), np.ascontiguousarray(w.T))
+    else:
+        fam, dt = nc.FAMILY_F32, nc.FLOAT32
+        base_a = rng.integers(-4, 4, (M, W), endpoint=True).astype(np.float32)
+        base_w = rng.integers(-4, 4, (N, W), endpoint=True).astype(np.float32)
+        a, w = base_a[:, a_off:a_off + K], base_w[:, b_off:b_off + K]
+        want = (a.astype(np.int64) @ w.T.astype(np.int64)).astype(np.float32)  # exact
+    Av = ctx.from_numpy(base_a).slice(1, a_off, a_off + K)
+    Bv = ctx.from_numpy(base_w).slice(1, b_off, b_off + K).transpose()   # (K, N), rows of B^T contiguous
+    for cfg in range(nc.config_count(fam)):
+        O = ctx.array((M, N), dt, zeros=True)
+        nc.check(nc.matmul(Av, Bv, O, nc.METHOD_AUTO, cfg), "matmul B in place")
+        assert np.array_equal(O.numpy(), want), (name, a_off, b_off, cfg)
