@@ -463,7 +463,7 @@ __global__ void k_pack_digits(PackArgs a) {
 #pragma unroll
   for (int p = 0; p < P; p++)
     if (__ballot((w[p][0] | w[p][1] | w[p][2] | w[p][3]) != 0)) mask |= 1 << p;
-  if ((threadIdx.x & 63) == 0 && mask) flags_or(a.flags, mask);
+  if ((threadIdx.x & 63) == 0 && mask && a.flags) flags_or(a.flags, mask); // no masks: null
 }
 
 // ---- FP8 scales: row maxima of |x| as float bits, then exponents ----------

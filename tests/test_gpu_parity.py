@@ -684,3 +684,20 @@ def test_b_read_in_place(ctx, name, a_off, b_off):
         O = ctx.array((M, N), dt, zeros=True)
         nc.check(nc.matmul(Av, Bv, O, nc.METHOD_AUTO, cfg), "matmul B in place")
         assert np.array_equal(O.numpy(), want), (name, a_off, b_off, cfg)
+
+
+@pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_I8)))
+def test_int8_a_in_place_b_any_strides(ctx, cfg):
+    """int8 with A read in place and B a view strided in both dimensions (every
+    other column): B goes through the generic one-plane digit pack, which
+    must not touch plane masks when the call has none."""
+    rng = np.random.default_rng(900 + cfg)
+    M, K, N = 130, 256, 70
+    a = rng.integers(-128, 127, (M, K), dtype=np.int8, endpoint=True)
+    base_b = rng.integers(-128, 127, (K, 2 * N), dtype=np.int8, endpoint=True)
+    B = ctx.from_numpy(base_b).slice(1, 0, 2 * N, 2)                # (K, N), strides (2N, 2)
+    for odt, flags in [(np.int8, 0), (np.int32, nc.OPT_WIDEN)]:
+        O = ctx.array((M, N), nc.DTYPE_OF_NP[np.dtype(odt)], zeros=True)
+        nc.check(nc.matmul(ctx.from_numpy(a), B, O, nc.METHOD_AUTO, cfg, flags), "int8 strided B")
+        want = oracle.matmul_int(a, np.ascontiguousarray(base_b[:, ::2]), odt)
+        assert np.array_equal(O.numpy(), want), (cfg, odt)
