@@ -893,6 +893,8 @@ __global__ void __launch_bounds__(WGM *WGN * 64, 1) k_gemm(const GemmArgs gin) {
   GemmArgs g = gin; // local copy: the tile code captures it by reference
   g.A += (long long)blockIdx.z * g.a_zs; // strided batch: slice blockIdx.z
   g.B += (long long)blockIdx.z * g.b_zs;
+  g.Araw += (long long)blockIdx.z * g.a_zs; // (an operand read in place has its own slice stride in a_zs / b_zs)
+  g.Braw += (long long)blockIdx.z * g.b_zs;
   g.C = (unsigned char *)g.C + (long long)blockIdx.z * g.c_zs;
   // NS stages of fragments, filled straight from HBM by global_load_lds
   __shared__ u32x4 lds[TileShape<MF, BM, BN, KC, WGM, WGN, MODE, NS>::LDS_VEC];
@@ -1171,11 +1173,11 @@ static const GemmConfig kF16[] = {
 // stages, so that several blocks share a CU and one block's copies and
 // stores overlap another's MFMAs (no split-K: the slices fill the GPU)
 static const GemmConfig kStridedBatch[NUMC_FAMILY_COUNT][2] = {
-    {CFGB(MfF32, 128, 128, 16, 2, 2, EPI_F32, 1, 2), CFGB(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 2)},
+    {CFGD(MfF32, 128, 128, 16, 2, 2, EPI_F32, 1, 2), CFGD(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 2)},
     {CFGS(MfBF16, 128, 128, 16, 2, 2, MODE_SPLIT3, EPI_F32, 1, 2), CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 2)},
     {CFGS(MfFP8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 2), CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 2)},
     {CFGB(MfF64, 64, 64, 16, 2, 2, EPI_F64, 1, 2), CFGB(MfF64, 64, 64, 32, 2, 2, EPI_F64, 1, 2)},
-    {CFGB(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 2), CFGB(MfI8, 64, 64, 64, 2, 2, EPI_INT, 1, 2)},
+    {CFGDA(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 2), CFGDA(MfI8, 64, 64, 64, 2, 2, EPI_INT, 1, 2)},
     {CFGB(MfBF16, 128, 128, 32, 2, 2, EPI_BF16, 1, 2), CFGB(MfBF16, 64, 64, 64, 2, 2, EPI_BF16, 1, 2)},
     {CFGB(MfF16, 128, 128, 32, 2, 2, EPI_F16, 1, 2), CFGB(MfF16, 64, 64, 64, 2, 2, EPI_F16, 1, 2)},
 };
@@ -1186,6 +1188,9 @@ const GemmConfig *numc_strided_batch_config(int family, int M, int N) {
 }
 
 #undef CFGB
+#undef CFGD
+#undef CFGDA
+#undef CFGIP
 
 struct FamilyTable {
   const GemmConfig *cfg;
@@ -1915,14 +1920,17 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // A read in place by the GEMM (no pack of A): FP32 and single-plane int8,
   // A's rows k-contiguous and 16-byte aligned, K a multiple of the k-chunk
   // (every copy stays inside its row)
-  const bool inplace_ok = nz == 1 && c.inplace[0][0] && !masks && p.K % c.cfg.kc == 0 &&
+  // (a strided batch: every slice 16-byte aligned too)
+  const bool inplace_ok = c.inplace[0][0] && !masks && p.K % c.cfg.kc == 0 &&
                           (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8);
   const bool adir = inplace_ok && p.csa == 1 && ((uintptr_t)p.A & 15) == 0 &&
-                    ((long long)p.rsa * (long long)ein) % 16 == 0;
+                    ((long long)p.rsa * (long long)ein) % 16 == 0 &&
+                    (nz == 1 || ((long long)p.a_bs * (long long)ein) % 16 == 0);
   // B likewise when the rows of B^T are k-contiguous (B stored transposed):
   // FP32 only
   const bool bdir = inplace_ok && c.inplace[1][0] && p.rsb == 1 && ((uintptr_t)p.B & 15) == 0 &&
-                    ((long long)p.csb * (long long)ein) % 16 == 0;
+                    ((long long)p.csb * (long long)ein) % 16 == 0 &&
+                    (nz == 1 || ((long long)p.b_bs * (long long)ein) % 16 == 0);
   if (masks)
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * NUMC_FLAG_INTS)), dim3(256), 0, st, (uint32_t *)flags,
                        (long long)(2 * NUMC_FLAG_INTS), 0u);
@@ -1942,9 +1950,9 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   if (adir && bdir) {
     // nothing to pack
   } else if (adir) {
-    launch_pack(family, p.in_dt, planes, pb, 1u, st);
+    launch_pack(family, p.in_dt, planes, pb, (unsigned)nz, st);
   } else if (bdir) {
-    launch_pack(family, p.in_dt, planes, pk, 1u, st);
+    launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
   } else if (pair) {
     const long long na = numc_batch_pack_blocks(family, pk), nb = numc_batch_pack_blocks(family, pb);
     const dim3 grid((unsigned)(na + nb));
@@ -1981,8 +1989,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.pa[0] = g.pb[0] = 0;
   g.bias = p.bias;
   g.bias_s = p.bias_s;
-  g.a_zs = pla * planes;
-  g.b_zs = plb * planes;
+  g.a_zs = adir ? p.a_bs * (long long)ein : pla * planes;
+  g.b_zs = bdir ? p.b_bs * (long long)ein : plb * planes;
   g.c_zs = p.c_bs * (long long)eout;
   g.Araw = (const unsigned char *)p.A;
   g.a_rs = (long long)p.rsa * (long long)ein;

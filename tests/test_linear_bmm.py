@@ -274,3 +274,29 @@ def test_bmm_transposed_a_and_strided_out(ctx):
     want, den = oracle.bmm(np.ascontiguousarray(np.swapaxes(at, 1, 2)), b)
     _check(full[:, :, 1:26], want, den, 30, np.float32, False)
     assert np.all(full[:, :, 0] == 5) and np.all(full[:, :, 26] == 5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt", [np.float32, np.int8], ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("b_transposed", [False, True], ids=["b-rows", "b-transposed"])
+def test_bmm_operands_in_place(ctx, dt, b_transposed):
+    """strided-batch bmm whose slices are 16-byte aligned with K = 64: float32
+    and int8 read a's slices in place, float32 also b's when b is given as a
+    transposed view (q @ k^T); exact sums against numpy's integer matmul"""
+    rng = np.random.default_rng(31 + b_transposed)
+    if dt == np.int8:
+        a = rng.integers(-128, 127, (3, 4, 100, 64), dtype=np.int8, endpoint=True)
+        kk = rng.integers(-128, 127, (3, 4, 90, 64), dtype=np.int8, endpoint=True)
+    else:
+        a = rng.integers(-4, 4, (3, 4, 100, 64), endpoint=True).astype(np.float32)
+        kk = rng.integers(-4, 4, (3, 4, 90, 64), endpoint=True).astype(np.float32)
+    bt = np.swapaxes(kk, 2, 3)                                        # (3, 4, 64, 90)
+    if b_transposed:
+        B = ctx.from_numpy(kk).transpose((0, 1, 3, 2))
+    else:
+        B = ctx.from_numpy(np.ascontiguousarray(bt))
+    exact = np.matmul(a.astype(np.int64), bt.astype(np.int64))
+    want = exact.astype(np.int8) if dt == np.int8 else exact.astype(np.float32)
+    O = ctx.array(want.shape, DT[dt], zeros=True)
+    nc.check(nc.bmm(ctx.from_numpy(a), B, O), "bmm in place")
+    assert np.array_equal(O.numpy(), want)
