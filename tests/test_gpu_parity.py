@@ -133,8 +133,8 @@ def test_f32_configs(ctx, cfg):
 
 @pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F32)))
 def test_f32_configs_strided(ctx, cfg):
-    """every FP32 configuration (packed and direct) on transposed / stepped
-    operand views and an output with gaps, ragged shapes"""
+    """every FP32 configuration on transposed / stepped operand views (both
+    packed) and an output with gaps, ragged shapes"""
     rng = np.random.default_rng(150 + cfg)
     for M, K, N in [(67, 45, 93), (130, 257, 33), (1, 19, 300)]:
         at = rng.standard_normal((K, M)).astype(np.float32)       # A stored transposed
