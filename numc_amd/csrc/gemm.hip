@@ -1647,10 +1647,7 @@ __global__ void __launch_bounds__(256) k_pack_raw16_rows(PackArgs a) {
 }
 
 // any other strides: one thread per fragment lane slot
-__global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
-  zslice(a);
-  const long long t = blockIdx.x * 256LL + threadIdx.x;
-  if (t >= (long long)a.nRT * a.nKF * 64) return;
+__device__ inline void raw16_unit(const PackArgs &a, long long t) {
   const int lane = (int)(t & 63);
   const long long fr = t >> 6;
   const int kf = (int)(fr % a.nKF);
@@ -1666,18 +1663,39 @@ __global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
   *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+__global__ void __launch_bounds__(256) k_pack_raw16(PackArgs a) {
+  zslice(a);
+  const long long t = blockIdx.x * 256LL + threadIdx.x;
+  if (t < (long long)a.nRT * a.nKF * 64) raw16_unit(a, t);
+}
+
 // ---------------------------------------------------------------------------
 // batched pack: every operand of every problem in one launch.  Operand i owns
 // blocks [prefix[i], prefix[i+1]) and owner[b] names the operand of block b
 // (one load, where a search of prefix would take a chain of dependent ones);
 // its `mode` (set on the host) says which tile walk it takes: 0 k-contiguous
 // tiles, 1 row-contiguous tiles, 2 one thread per 16-byte lane slot (any
-// other strides), 3/4 the byte tiles of int8 operands.
+// other strides), 3/4 the byte tiles of int8 operands, 5/6/7 the raw 16-bit
+// walks (k-contiguous, row-contiguous, other strides).
 // ---------------------------------------------------------------------------
 // block u of operand a, by the walk a.mode names
 template <int FAM>
 __device__ __attribute__((always_inline)) inline void pack_walk(const PackArgs &a, long long u, uint32_t *tile) {
   constexpr int MODE_PK = FAM == NUMC_FAMILY_F32 ? 0 : FAM == NUMC_FAMILY_FP8 ? 2 : 1;
+  if constexpr (FAM == NUMC_FAMILY_BF16 || FAM == NUMC_FAMILY_F16) { // 16-bit values copied as they are
+    if (a.mode == 5) {
+      PackArgs b = a; // the k-contiguous rows as bytes
+      b.K *= 2;
+      b.rs *= 2;
+      bytes_tile_k(b, u, tile);
+    } else if (a.mode == 6) {
+      raw16_tile_rows(a, u, tile);
+    } else {
+      const long long t = u * 256 + threadIdx.x;
+      if (t < (long long)a.nRT * a.nKF * 64) raw16_unit(a, t);
+    }
+    return;
+  }
   if constexpr (FAM == NUMC_FAMILY_I8) {
     if (a.mode == 3) {
       bytes_tile_k(a, u, tile);
@@ -1727,6 +1745,18 @@ __global__ void __launch_bounds__(256) k_pack_pair(const PackArgs a, const PackA
 
 // blocks the batched pack gives one operand, and the walk it takes
 int numc_batch_pack_blocks(int family, PackArgs &pk) {
+  if (family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16) { // 5/6/7: the raw 16-bit walks
+    if (pk.ks == 1) {
+      pk.mode = 5;
+      return (int)(pk.nRT * cdiv(pk.nKF, 16));
+    }
+    if (pk.rs == 1) {
+      pk.mode = 6;
+      return (int)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4));
+    }
+    pk.mode = 7;
+    return (int)cdiv((long long)pk.nRT * pk.nKF * 64, 256);
+  }
   const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
   if (family == NUMC_FAMILY_I8 && pk.ks == 1) { // int8 operands, one plane: byte tiles
     pk.mode = 3;
@@ -1914,7 +1944,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // family.  A single int8 digit plane needs no plane masks (they only let
   // the GEMM skip empty plane products), so that case sets none.
   const bool pair = nz == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
-                                family == NUMC_FAMILY_FP8 ||
+                                family == NUMC_FAMILY_FP8 || family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16 ||
                                 (family == NUMC_FAMILY_I8 && planes == 1 && p.in_dt == NUMC_DTYPE_INT8));
   const bool masks = family == NUMC_FAMILY_I8 && !pair;
   // A read in place by the GEMM (no pack of A): FP32 and single-plane int8,
@@ -1959,6 +1989,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F32>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_SPLIT_BF16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else if (family == NUMC_FAMILY_FP8) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_FP8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+    else if (family == NUMC_FAMILY_BF16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_BF16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+    else if (family == NUMC_FAMILY_F16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_I8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
   } else {
     launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
