@@ -701,3 +701,21 @@ def test_int8_a_in_place_b_any_strides(ctx, cfg):
         nc.check(nc.matmul(ctx.from_numpy(a), B, O, nc.METHOD_AUTO, cfg, flags), "int8 strided B")
         want = oracle.matmul_int(a, np.ascontiguousarray(base_b[:, ::2]), odt)
         assert np.array_equal(O.numpy(), want), (cfg, odt)
+
+
+def test_device_copy(ctx):
+    """numc_device_copy: exact bytes for sizes around its four-loads-per-thread
+    stride (ragged tails), and its argument checks"""
+    rng = np.random.default_rng(41)
+    for nbytes in [16, 48, 4096 * 16 + 32, (1 << 23) + 16 * 3, 2048 * 256 * 16 * 4 + 16 * 5]:
+        x = rng.integers(0, 256, nbytes, dtype=np.uint8)
+        X = ctx.from_numpy(x)
+        Y = ctx.array((nbytes,), nc.UINT8, zeros=True)
+        X.to_device()
+        Y.to_device()
+        src, dst = nc.lib().numc_array_device_data(X.h), nc.lib().numc_array_device_data(Y.h)
+        nc.check(nc.lib().numc_device_copy(ctx.h, dst, src, nbytes), "numc_device_copy")
+        Y.to_host()
+        assert np.array_equal(Y.numpy(), x), nbytes
+    assert nc.lib().numc_device_copy(ctx.h, dst, src, 24) == nc.ERR_VALUE
+    assert nc.lib().numc_device_copy(ctx.h, None, src, 16) == nc.ERR_NULL
