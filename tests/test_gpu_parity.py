@@ -704,8 +704,8 @@ def test_int8_a_in_place_b_any_strides(ctx, cfg):
 
 
 def test_device_copy(ctx):
-    """numc_device_copy: exact bytes for sizes around its four-loads-per-thread
-    stride (ragged tails), and its argument checks"""
+    """numc_device_copy: exact bytes for sizes below, at and past its grid
+    stride (2048 blocks x 256 threads x 16 bytes), and its argument checks"""
     rng = np.random.default_rng(41)
     for nbytes in [16, 48, 4096 * 16 + 32, (1 << 23) + 16 * 3, 2048 * 256 * 16 * 4 + 16 * 5]:
         x = rng.integers(0, 256, nbytes, dtype=np.uint8)
