@@ -118,11 +118,16 @@ def cpu_baseline(n, seed, cores):
     return json.loads(r.stdout.strip().splitlines()[-1]), None
 
 
+def batch_shapes(count, seed):
+    """(M, N, K) of the uneven batch: each a log-uniform integer in [16, 1024]"""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.round(np.exp(rng.uniform(np.log(16), np.log(1024), (count, 3)))), 16, 1024).astype(int)
+
+
 def bench_batched(nc, count, seed, reps):
     """4096 problems, M, N, K log-uniform integers in [16, 1024], U[-1,1) values:
     one persistent graph-replayed launch vs one numc_matmul_ex call per problem."""
-    rng = np.random.default_rng(seed)
-    dims = np.clip(np.round(np.exp(rng.uniform(np.log(16), np.log(1024), (count, 3)))), 16, 1024).astype(int)
+    dims = batch_shapes(count, seed)
     ctx = nc.Ctx()
     ctx.set_sync(False)
     res = {"problems": count, "flop": float(np.sum(2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]))}

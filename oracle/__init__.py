@@ -46,6 +46,7 @@ def lib():
         L.orc_split_bf16.argtypes = [_P, C.c_size_t, _P, _P]
         L.orc_e4m3_rne.argtypes = [C.c_float]
         L.orc_e4m3_rne.restype = C.c_uint8
+        L.orc_e4m3_rne_n.argtypes = [_P, C.c_size_t, _P]
         L.orc_e4m3_to_f32.argtypes = [C.c_uint8]
         L.orc_e4m3_to_f32.restype = C.c_float
         L.orc_fp8_scale_exp.argtypes = [C.c_float]
@@ -173,16 +174,32 @@ def fp8_scale_exp(m):
     return lib().orc_fp8_scale_exp(float(m))
 
 
+def e4m3_array(x):
+    """orc_e4m3_rne over every element of x (same shape, uint8 codes)"""
+    x = np.ascontiguousarray(x, np.float32)
+    q = np.empty(x.shape, np.uint8)
+    lib().orc_e4m3_rne_n(_ptr(x), x.size, _ptr(q))
+    return q
+
+
+def _e4m3_table():
+    return np.array([lib().orc_e4m3_to_f32(q) for q in range(256)], np.float64)
+
+
+def fp8_quantize_rows(x):
+    """Rows of x scaled by 2^e (e = fp8_scale_exp of the row's max |x|), rounded
+    to e4m3, decoded and unscaled back to float64: the exact operand rows the
+    FP8 method multiplies.  Columns of B are the rows of B^T."""
+    x = np.asarray(x, np.float32)
+    e = np.array([fp8_scale_exp(np.max(np.abs(r)) if r.size else 0.0) for r in x], np.int64)
+    q = _e4m3_table()[e4m3_array(np.ldexp(x, e[:, None]).astype(np.float32))]
+    return np.ldexp(q, -e[:, None])
+
+
 def fp8_quantized_inputs(a, b):
     """Per-row scaled e4m3 A, per-column scaled e4m3 B, decoded and unscaled back to
     float64: the exact operands the FP8 method multiplies."""
-    table = np.array([lib().orc_e4m3_to_f32(q) for q in range(256)], np.float64)
-    enc = np.vectorize(lambda v: lib().orc_e4m3_rne(float(v)), otypes=[np.uint8])
-    ea = np.array([fp8_scale_exp(np.max(np.abs(r)) if r.size else 0.0) for r in a], np.int64)
-    eb = np.array([fp8_scale_exp(np.max(np.abs(c)) if c.size else 0.0) for c in b.T], np.int64)
-    qa = table[enc(np.ldexp(a.astype(np.float32), ea[:, None]).astype(np.float32))]
-    qb = table[enc(np.ldexp(b.astype(np.float32), eb[None, :]).astype(np.float32))]
-    return np.ldexp(qa, -ea[:, None]), np.ldexp(qb, -eb[None, :])
+    return fp8_quantize_rows(a), fp8_quantize_rows(np.asarray(b).T).T
 
 
 # ---- GEMM consumers of numc's roadmap (ROADMAP.md:139-140) --------------------

@@ -347,6 +347,10 @@ uint8_t orc_e4m3_rne(float x) {
   return (uint8_t)(sign | code);
 }
 
+void orc_e4m3_rne_n(const float *x, size_t n, uint8_t *q) {
+  for (size_t i = 0; i < n; i++) q[i] = orc_e4m3_rne(x[i]);
+}
+
 float orc_e4m3_to_f32(uint8_t q) {
   int s = q >> 7, e = (q >> 3) & 0xF, m = q & 7;
   float v;

@@ -109,6 +109,7 @@ uint16_t orc_bf16_rne(float x);
 float orc_bf16_to_f32(uint16_t h);
 void orc_split_bf16(const float *x, size_t n, uint16_t *hi, uint16_t *lo);
 uint8_t orc_e4m3_rne(float x);        /* OCP e4m3fn, saturating to +-448 */
+void orc_e4m3_rne_n(const float *x, size_t n, uint8_t *q); /* the same over an array */
 float orc_e4m3_to_f32(uint8_t q);
 /* exponent e such that the row maximum times 2^e lies in [128, 256);
  * 0 for an all-zero row. */

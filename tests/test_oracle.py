@@ -182,6 +182,25 @@ def test_e4m3_against_torch():
     assert oracle.e4m3(np.float32([500.0, -1e30]))[0] == 0x7E
 
 
+def test_e4m3_array_equals_scalar_and_quantized_rows():
+    """the array encoder is the scalar one element by element (every finite
+    bf16-representable magnitude plus random values), and the row quantizer
+    scales each row's maximum into [128, 256) before rounding"""
+    hi = np.arange(0, 1 << 16, dtype=np.uint32)
+    x = np.concatenate([(hi << 16).view(np.float32),
+                        np.random.default_rng(2).standard_normal(5000).astype(np.float32)])
+    x = x[np.isfinite(x)]
+    assert np.array_equal(oracle.e4m3_array(x), oracle.e4m3(x))
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal((7, 33)) * 2.0 ** rng.integers(-20, 20, (7, 1))).astype(np.float32)
+    q = oracle.fp8_quantize_rows(a)
+    for r in range(a.shape[0]):
+        e = oracle.fp8_scale_exp(np.max(np.abs(a[r])))
+        assert 128.0 <= np.max(np.abs(a[r])) * 2.0 ** e < 256.0
+        want = oracle.e4m3_decode(oracle.e4m3(np.ldexp(a[r], e).astype(np.float32))).astype(np.float64)
+        assert np.array_equal(q[r], np.ldexp(want, -e))
+
+
 def test_fp8_scale_exp_closed_form():
     for m in [1e-30, 3e-5, 0.5, 1.0, 1.7, 127.9, 128.0, 255.9, 256.0, 3e20]:
         e = oracle.fp8_scale_exp(m)
