@@ -124,17 +124,184 @@ def batch_shapes(count, seed):
     return np.clip(np.round(np.exp(rng.uniform(np.log(16), np.log(1024), (count, 3)))), 16, 1024).astype(int)
 
 
-def bench_batched(nc, count, seed, reps):
-    """4096 problems, M, N, K log-uniform integers in [16, 1024], U[-1,1) values:
-    one persistent graph-replayed launch vs one numc_matmul_ex call per problem."""
+def host_cpu():
+    """model, ISA and core counts of this host (where cpu_baseline runs)"""
+    info = {"model": None, "logical_cpus": os.cpu_count(), "physical_cores": None,
+            "isa_flags": None, "numc_build": "x86-64-v3 (AVX2 + FMA), -mno-avx512f, OpenMP"}
+    flags = set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and info["model"] is None:
+                info["model"] = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    info["isa_flags"] = sorted(f for f in flags if f in ("avx", "avx2", "fma", "avx512f", "avx512bw", "avx512vnni",
+                                                          "avx_vnni", "amx_tile"))
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        cps, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sockets:
+            info["physical_cores"] = int(cps) * int(sockets)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def _work_unit(unit):
+    return "op" if unit.startswith("TOP") else "flop"
+
+
+def roofline_entry(work, nbytes, kernel_ms, peak, unit, kernel):
+    """roofline of one kernel: bound = whichever of the MFMA peak and HBM takes
+    longer for the algorithmic work and bytes; achieved = that quantity / the
+    kernel's average duration"""
+    t_mfma, t_hbm = work / (peak * 1e12), nbytes / (HBM_PEAK * 1e9)
+    if t_mfma >= t_hbm:
+        ach = work / kernel_ms * 1e-9
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": unit,
+                "frac": round(ach / peak, 4), "traffic": None, "kernel": kernel,
+                "algorithmic": "%.4g %s per launch (%.4g B compulsory)" % (work, _work_unit(unit), nbytes)}
+    ach = nbytes / kernel_ms * 1e-6
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK, 4), "traffic": None, "kernel": kernel,
+            "algorithmic": "%.4g B compulsory per launch (%.4g %s)" % (nbytes, work, _work_unit(unit))}
+
+
+def run_ref_bench(args, cores, stdin=None, timeout=300):
+    import oracle
+    if not os.path.exists(oracle.REF_BENCH):
+        return None, "oracle/_ref not built"
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    try:
+        r = subprocess.run([oracle.REF_BENCH] + [str(x) for x in args], input=stdin, capture_output=True, text=True,
+                           env=env, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return None, "numc_ref_bench timed out"
+    if r.returncode != 0:
+        return None, "numc_ref_bench failed (%d): %s" % (r.returncode, r.stderr[-200:])
+    return json.loads(r.stdout.strip().splitlines()[-1]), None
+
+
+def time_whole_and_kernels(nc, ctx, fn, family, steps, warmup, batched=False):
+    """whole-call ms (HIP events on the context stream, no per-kernel events),
+    then the same calls with per-kernel events: average pack / GEMM / split-K
+    reduce (or batched GEMM) durations"""
+    t_end = time.perf_counter() + WARM_S
+    while time.perf_counter() < t_end:
+        nc.check(fn(), "clock warm-up")
+        ctx.synchronize()
+    for _ in range(warmup):
+        nc.check(fn(), "warmup")
+    ctx.synchronize()
+    ctx.mark(0)
+    for _ in range(steps):
+        nc.check(fn(), "step")
+    ctx.mark(1)
+    ctx.synchronize()
+    whole = ctx.elapsed_ms(0, 1) / steps
+    ctx.profile(True)
+    for cat in (0, 1, 2, 3):
+        ctx.profile_read(cat, family)
+    for _ in range(steps):
+        nc.check(fn(), "profiled step")
+    ctx.synchronize()
+    k = {}
+    for cat, name in ((0, "pack"), (1, "gemm"), (2, "reduce"), (3, "batched_gemm")):
+        ms, n = ctx.profile_read(cat, family)
+        k[name] = (ms / n if n else 0.0, n / steps)
+    ctx.profile(False)
+    main = k["batched_gemm"] if batched else k["gemm"]
+    return {"ms_per_call": whole, "kernel_ms": main[0], "kernel_launches_per_call": main[1],
+            "pack_ms": k["pack"][0], "pack_launches_per_call": k["pack"][1],
+            "reduce_ms": k["reduce"][0], "reduce_launches_per_call": k["reduce"][1]}
+
+
+# the single-problem configurations (numc_matmul on HBM-resident operands):
+# (key, dtype, numc dtype code, n, dist, ref-bench dist, lo, hi, seed, family, peak, unit, bytes/elem in, out)
+SINGLE_CONFIGS = [
+    ("c1_f32_512", "FLOAT32", 8, 512, "DIST_UNIFORM", "u", -1, 1, 1, "FAMILY_F32", 157.3, "TFLOP/s", 4, 4),
+    ("c2_f64_2048", "FLOAT64", 9, 2048, "DIST_NORMAL", "n", 0, 0, 2, "FAMILY_F64", 78.6, "TFLOP/s", 8, 8),
+    ("c2_i32_2048", "INT32", 2, 2048, "DIST_INT", "i", -100, 100, 2, "FAMILY_I8", 5030.0, "TOP/s", 4, 4),
+    ("c2_i8_4096", "INT8", 0, 4096, "DIST_INT", "i", -128, 127, 2, "FAMILY_I8", 5030.0, "TOP/s", 1, 1),
+]
+
+
+def bench_configs(nc, steps, warmup, cores, with_cpu):
+    """The other configurations of the metric, each with its whole-call rate,
+    its GEMM kernel's roofline and numc's own CPU numc_matmul on the same bytes."""
+    res = {}
+    for key, dtn, code, n, dist, rdist, lo, hi, seed, famn, peak, unit, ein, eout in SINGLE_CONFIGS:
+        dt, fam = getattr(nc, dtn), getattr(nc, famn)
+        ctx = nc.Ctx()
+        ctx.set_sync(False)
+        A, B, Cm = ctx.array((n, n), dt), ctx.array((n, n), dt), ctx.array((n, n), dt, zeros=True)
+        A.fill_random(getattr(nc, dist), seed, 0, lo, hi)   # host fill: the bytes numc's CPU run gets
+        B.fill_random(getattr(nc, dist), seed, 1, lo, hi)
+        sums = (str(wsum(A.numpy())), str(wsum(B.numpy())))
+        for x in (A, B, Cm):
+            x.to_device()
+        small = n <= 1024
+        st = max(steps, 200) if small else max(steps, 20)
+        t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul(A, B, Cm), fam, st, warmup)
+        cfg = nc.lib().numc_selector_choice(fam, n, n, n, 0)
+        work, nbytes = 2.0 * n ** 3, float(n * n * (2 * ein + eout))
+        e = {"workload": "%s %d^3, numc_matmul, operands resident in HBM" % (dtn.lower(), n),
+             "steps": st, "ms_per_call": round(t["ms_per_call"], 5),
+             "rate_whole_call": round(work / t["ms_per_call"] * 1e-9, 3), "unit": unit,
+             "kernel_ms": round(t["kernel_ms"], 5), "kernel_launches_per_call": t["kernel_launches_per_call"],
+             "pack_ms": round(t["pack_ms"], 5), "pack_launches_per_call": t["pack_launches_per_call"],
+             "reduce_ms": round(t["reduce_ms"], 5), "reduce_launches_per_call": t["reduce_launches_per_call"],
+             "config": nc.config_get(fam, cfg) if cfg >= 0 else None,
+             "roofline": roofline_entry(work, nbytes, t["kernel_ms"], peak, unit,
+                                        "%s GEMM kernel (avg of %d launches)" % (famn[7:].lower(), st))}
+        log("bench: %s %.4f ms/call (kernel %.4f ms, roofline frac %.3f)" %
+            (key, t["ms_per_call"], t["kernel_ms"], e["roofline"]["frac"]))
+        ctx.close()
+        if with_cpu:
+            reps = 50 if small else (3 if n >= 4096 else 5)
+            r, why = run_ref_bench([code, n, n, n, rdist, seed, 1, reps] + ([lo, hi] if rdist != "n" else []), cores)
+            if r:
+                same = (r["a_wsum"], r["b_wsum"]) == sums
+                e["cpu_baseline"] = {"value": round(r["gflops"] * 1e-3, 5), "unit": unit, "cores": r["threads"],
+                                     "kind": "reference", "ms": r["ms_min"],
+                                     "sample": "numc_matmul %s %d^3 (numc AVX2+OpenMP build), min of %d after 1 "
+                                               "warmup, same input bytes: %s%s" %
+                                               (dtn.lower(), n, reps, same,
+                                                "; numc's packed int8 path saturates per 512-k block, so its "
+                                                "outputs differ (times comparable)" if code == 0 else "")}
+            else:
+                e["cpu_baseline"] = {"value": None, "unit": unit, "cores": cores, "kind": "reference", "sample": why}
+        res[key] = e
+    return res
+
+
+def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=True):
+    """4096 problems, M, N, K log-uniform integers in [16, 1024], U[-1,1) values
+    (int8: integers in [-128, 127]): one persistent graph-replayed launch vs one
+    numc_matmul_ex call per problem, the batched GEMM kernel's roofline, and
+    numc's CPU numc_matmul over the same problems (numc has no batched GEMM)."""
     dims = batch_shapes(count, seed)
     ctx = nc.Ctx()
     ctx.set_sync(False)
-    res = {"problems": count, "flop": float(np.sum(2.0 * dims[:, 0] * dims[:, 1] * dims[:, 2]))}
-    for label, dt, odt, method, flags in (("f32", nc.FLOAT32, nc.FLOAT32, nc.METHOD_PLAIN, 0),
-                                          ("split_bf16", nc.FLOAT32, nc.FLOAT32, nc.METHOD_SPLIT_BF16, 0),
-                                          ("int8_to_int32", nc.INT8, nc.INT32, nc.METHOD_AUTO, nc.OPT_WIDEN)):
+    M_, N_, K_ = dims[:, 0].astype(float), dims[:, 1].astype(float), dims[:, 2].astype(float)
+    flop = float(np.sum(2.0 * M_ * N_ * K_))
+    res = {"problems": count, "flop": flop}
+    cpu = {}
+    if with_cpu:
+        shapes = "\n".join("%d %d %d" % tuple(d) for d in dims)
+        for code, lo, hi in ((8, -1, 1), (0, -128, 127)):
+            r, why = run_ref_bench(["batch", code, seed, 1, 2, lo, hi], cores, stdin=shapes)
+            cpu[code] = (r, why)
+    for label, dt, odt, method, flags, fam, peak, unit, ein, eout in (
+            ("f32", nc.FLOAT32, nc.FLOAT32, nc.METHOD_PLAIN, 0, nc.FAMILY_F32, 157.3, "TFLOP/s", 4, 4),
+            ("split_bf16", nc.FLOAT32, nc.FLOAT32, nc.METHOD_SPLIT_BF16, 0, nc.FAMILY_SPLIT_BF16, 2516.0 / 3,
+             "TFLOP/s", 4, 4),
+            ("int8_to_int32", nc.INT8, nc.INT32, nc.METHOD_AUTO, nc.OPT_WIDEN, nc.FAMILY_I8, 5030.0, "TOP/s", 1, 4)):
         As, Bs, Os = [], [], []
+        in_sum = 0
         for i, (M, N, K) in enumerate(dims):
             A = ctx.array((M, K), dt)
             B = ctx.array((K, N), dt)
@@ -144,6 +311,9 @@ def bench_batched(nc, count, seed, reps):
             else:
                 A.fill_random(nc.DIST_UNIFORM, seed, 2 * i, -1, 1)
                 B.fill_random(nc.DIST_UNIFORM, seed, 2 * i + 1, -1, 1)
+            if with_cpu:
+                in_sum += int(A.numpy().view(np.uint8).sum(dtype=np.uint64)) + \
+                    int(B.numpy().view(np.uint8).sum(dtype=np.uint64))
             O = ctx.array((M, N), odt, zeros=True)
             for x in (A, B, O):
                 x.to_device()
@@ -161,6 +331,8 @@ def bench_batched(nc, count, seed, reps):
             nc.check(nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), "batched")
         ctx.mark(3)
         t_batched = ctx.elapsed_ms(2, 3) / reps
+        t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), fam,
+                                   max(2, steps // 2), 1, batched=True)
         # one call per problem, same stream, no graph
         for i in range(count):
             nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate warmup")
@@ -170,13 +342,35 @@ def bench_batched(nc, count, seed, reps):
             nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate")
         ctx.mark(5)
         t_sep = ctx.elapsed_ms(4, 5)
-        res[label] = {"ms_batched": round(t_batched, 4), "tflops_batched": round(res["flop"] / t_batched * 1e-9, 2),
-                      "tile_threshold": thresh,
-                      "ms_separate_launches": round(t_sep, 3),
-                      "tflops_separate": round(res["flop"] / t_sep * 1e-9, 2),
-                      "speedup": round(t_sep / t_batched, 2)}
-        log("bench: batched %s: %.3f ms (%.1f TFLOP/s) vs %d separate calls %.2f ms" %
-            (label, t_batched, res["flop"] / t_batched * 1e-9, count, t_sep))
+        nbytes = float(np.sum(ein * (M_ * K_ + K_ * N_) + eout * M_ * N_))
+        e = {"ms_batched": round(t_batched, 4), "tflops_batched": round(flop / t_batched * 1e-9, 2),
+             "tile_threshold": thresh,
+             "ms_separate_launches": round(t_sep, 3),
+             "tflops_separate": round(flop / t_sep * 1e-9, 2),
+             "speedup": round(t_sep / t_batched, 2),
+             "kernel_ms": round(t["kernel_ms"], 4), "pack_ms": round(t["pack_ms"], 4),
+             "pack_launches_per_call": t["pack_launches_per_call"],
+             "roofline": roofline_entry(flop, nbytes, t["kernel_ms"], peak, unit,
+                                        "persistent batched GEMM kernel (avg of %d launches)" % max(2, steps // 2))}
+        if with_cpu:
+            r, why = cpu[0 if dt == nc.INT8 else 8]
+            if r:
+                e["cpu_baseline"] = {
+                    "value": round(r["gflops"] * 1e-3, 5), "unit": unit, "cores": r["threads"], "kind": "reference",
+                    "ms": r["ms_min"],
+                    "sample": "numc_matmul per problem over the %d problems (%s; numc AVX2+OpenMP build), min of 2 "
+                              "passes after 1, same input bytes: %s%s" %
+                              (count, "float32" if dt != nc.INT8 else "int8 in, int8 out",
+                               r["in_byte_sum"] == str(in_sum),
+                               "; numc has no split-BF16, its float32 GEMM is the CPU method" if label == "split_bf16"
+                               else "; numc has no int8->int32 GEMM and its packed int8 path saturates, so times are "
+                                    "comparable, outputs are not" if dt == nc.INT8 else "")}
+            else:
+                e["cpu_baseline"] = {"value": None, "unit": unit, "cores": cores, "kind": "reference", "sample": why}
+        res[label] = e
+        log("bench: batched %s: %.3f ms (%.1f T/s), kernel %.3f ms, pack %.3f ms, vs %d separate calls %.2f ms" %
+            (label, t_batched, flop / t_batched * 1e-9, t["kernel_ms"], t["pack_ms"], count, t_sep))
+        del As, Bs, Os
     ctx.close()
     return res
 
@@ -356,10 +550,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (f32 512^3, f64/int32 2048^3, int8 4096^3)")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--only", default=None,
-                    help="time one method: split_bf16, f32 or fp8; or 'batched' for the uneven batch alone")
+                    help="time one method: split_bf16, f32 or fp8; 'batched' for the uneven batch alone; "
+                         "'configs' for the other configurations alone")
     ap.add_argument("--suite", action="store_true",
                     help="time the other configurations (f32 512^3, f64 2048^3, int32 2048^3, int8 4096^3, "
                          "bf16/f16 8192^3, numc_linear, numc_bmm)")
@@ -394,9 +591,15 @@ def main():
     if args.suite:
         bench_suite(nc, args.steps, args.warmup, args.seed)
         return
+    cores = min(16, os.cpu_count() or 1)
     if args.only == "batched":
         log("bench: batched uneven GEMM, %d problems" % args.batch)
-        print(json.dumps(bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2))), flush=True)
+        print(json.dumps(bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2), args.steps,
+                                       args.warmup, cores, not args.no_cpu)), flush=True)
+        return
+    if args.only == "configs":
+        print(json.dumps({"configs": bench_configs(nc, args.steps, args.warmup, cores, not args.no_cpu),
+                          "host_cpu": host_cpu()}), flush=True)
         return
     n = args.n
     flops = 2.0 * n * n * n
@@ -465,7 +668,6 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (numc AVX2 numc_matmul, %d threads)" % cores)
         res, why = cpu_baseline(n, args.seed, cores)
         if res is not None:
@@ -495,10 +697,14 @@ def main():
         log("bench: host-resident split-BF16 call %.2f ms (%.1f TFLOP/s incl. PCIe)" % (ms, flops / ms * 1e-9))
         ctx_h.close()
 
-    batched = None
+    configs = {}
+    if not args.no_configs:
+        log("bench: the other configurations")
+        configs = bench_configs(nc, args.steps, args.warmup, cores, not args.no_cpu)
     if not args.no_batched:
         log("bench: batched uneven GEMM, %d problems" % args.batch)
-        batched = bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2))
+        configs["c4_batch_%d" % args.batch] = bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2),
+                                                            args.steps, args.warmup, cores, not args.no_cpu)
 
     s = variants["split_bf16"]
     p = variants["f32"]
@@ -534,8 +740,9 @@ def main():
         "variants": {k: {kk: (float("%.5g" % vv) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
-    if batched:
-        out["batched"] = batched
+    if configs:
+        out["configs"] = configs
+    out["host_cpu"] = host_cpu()
     if pcie:
         out["host_resident_pcie_inclusive"] = pcie
     if errs:

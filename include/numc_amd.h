@@ -91,8 +91,11 @@ NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, NumcArray *con
                                       NumcArray *const *b, NumcArray *const *out,
                                       const NumcMatmulOpts *opts, int reps);
 
-/* number of times the context replayed its cached batched graph */
+/* number of times the context replayed one of its cached batched graphs */
 NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx);
+/* graphs cached on the context: the last 4 distinct batches (by arrays,
+ * shapes, strides and schedule), least recently used replaced first */
+NUMC_API int numc_ctx_batch_graphs_cached(NumcCtx *ctx);
 
 /* ---- residency -------------------------------------------------------------
  * By default the host copy is the current one: every GPU call uploads its
@@ -119,8 +122,10 @@ NUMC_API int numc_ctx_timer_elapsed(NumcCtx *ctx, int slot0, int slot1, float *m
 
 /* Kernel timing: with profiling on, HIP events are recorded on the context
  * stream around every pack (category 0), GEMM (1), split-K reduce (2) and
- * batched (3) launch; read returns the summed milliseconds and launch count
- * of one category and family (family NUMC_FAMILY_COUNT = other kernels). */
+ * batched GEMM (3) launch; read returns the summed milliseconds and launch
+ * count of one category and family (family NUMC_FAMILY_COUNT = other
+ * kernels).  While profiling, numc_matmul_batched launches its cached batch's
+ * kernels one by one instead of replaying the graph, so each is timed. */
 NUMC_API int numc_ctx_profile(NumcCtx *ctx, int enable);
 NUMC_API int numc_ctx_profile_read(NumcCtx *ctx, int category, int family, double *total_ms,
                                    long long *launches, int reset);

@@ -120,6 +120,7 @@ _SIGS = {
     "numc_matmul_batched_tune": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                            C.POINTER(MatmulOpts), C.c_int]),
     "numc_ctx_batch_graph_replays": (C.c_longlong, [_P]),
+    "numc_ctx_batch_graphs_cached": (C.c_int, [_P]),
     "numc_autotune_release": (C.c_int, [_P]),
     "numc_device_copy": (C.c_int, [_P, _P, _P, _SZ]),
     "numc_selector_tune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -29,13 +29,29 @@ int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, cons
                     bool finish); // matmul.cpp
 
 namespace {
-struct BatchCache {
+// the launch a cached graph holds, kept to relaunch its kernels eagerly when
+// the context profiles kernels (events cannot time kernels inside a graph)
+struct BatchLaunch {
+  int family = 0;
+  const PackArgs *descs = nullptr;
+  const long long *prefix = nullptr;
+  const int *owner = nullptr;
+  long long units = 0;
+  const BatchProblem *probs = nullptr;
+  const int4 *items = nullptr;
+  int nitems = 0;
+  int *counter = nullptr;
+  int grid = 0;
+};
+
+struct BatchEntry {
   std::vector<uint64_t> key;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  void *dev = nullptr;
+  void *dev = nullptr; // tables and packed operands of this batch
   size_t dev_bytes = 0;
-  long long replays = 0;
+  BatchLaunch launch;
+  long long last_use = 0;
   void release() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -45,6 +61,29 @@ struct BatchCache {
     dev = nullptr;
     dev_bytes = 0;
     key.clear();
+  }
+};
+
+// the graphs of the last few distinct batches (least recently used goes first)
+constexpr int kBatchCacheEntries = 4;
+struct BatchCache {
+  BatchEntry entries[kBatchCacheEntries];
+  long long replays = 0, tick = 0;
+  void release() {
+    for (auto &e : entries) e.release();
+  }
+  BatchEntry *find(const std::vector<uint64_t> &key) {
+    for (auto &e : entries)
+      if (e.exec && e.key == key) return &e;
+    return nullptr;
+  }
+  BatchEntry *victim() {
+    BatchEntry *v = &entries[0];
+    for (auto &e : entries) {
+      if (!e.exec) return &e;
+      if (e.last_use < v->last_use) v = &e;
+    }
+    return v;
   }
 };
 
@@ -73,6 +112,13 @@ extern "C" void numc_batch_cache_free(NumcCtx *ctx) {
 
 extern "C" NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx) {
   return ctx && ctx->batch_cache ? ((BatchCache *)ctx->batch_cache)->replays : 0;
+}
+
+extern "C" NUMC_API int numc_ctx_batch_graphs_cached(NumcCtx *ctx) {
+  if (!ctx || !ctx->batch_cache) return 0;
+  int n = 0;
+  for (auto &e : ((BatchCache *)ctx->batch_cache)->entries) n += e.exec != nullptr;
+  return n;
 }
 
 extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArray *const *a,
@@ -164,8 +210,8 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   if (!ctx->batch_cache) ctx->batch_cache = new BatchCache();
   BatchCache *cache = (BatchCache *)ctx->batch_cache;
 
-  bool have_graph = cache->exec && key == cache->key;
-  if (!have_graph) {
+  BatchEntry *entry = cache->find(key);
+  if (!entry) {
     // ---- plan: tile class and packed layout per problem ----
     long long packed = 0;
     for (size_t i = 0; i < count; i++) {
@@ -242,11 +288,12 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
       size_t o_pack = align_up(o_cnt + 256, 256);
       size_t total = o_pack + (size_t)packed + 256;
+      BatchEntry *e = cache->victim();
       HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-      cache->release();
-      HIP_TRY(hipMalloc(&cache->dev, total), "hipMalloc(batched workspace)");
-      cache->dev_bytes = total;
-      unsigned char *base = (unsigned char *)cache->dev;
+      e->release();
+      HIP_TRY(hipMalloc(&e->dev, total), "hipMalloc(batched workspace)");
+      e->dev_bytes = total;
+      unsigned char *base = (unsigned char *)e->dev;
       // fill in device addresses now that the workspace exists
       size_t di = 0;
       for (size_t i = 0; i < count; i++) {
@@ -281,37 +328,54 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       int cus = 256;
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
       const int grid = std::max(1, std::min(nitems, numc_batch_occupancy(family) * cus));
+      BatchLaunch &L = e->launch;
+      L.family = family;
+      L.descs = (const PackArgs *)(base + o_desc);
+      L.prefix = (const long long *)(base + o_pref);
+      L.owner = (const int *)(base + o_own);
+      L.units = prefix.back();
+      L.probs = (const BatchProblem *)(base + o_prob);
+      L.items = (const int4 *)(base + o_item);
+      L.nitems = nitems;
+      L.counter = (int *)(base + o_cnt);
+      L.grid = grid;
       HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      int lerr = numc_batch_launch(family, (const PackArgs *)(base + o_desc), (const long long *)(base + o_pref),
-                                   (const int *)(base + o_own), prefix.back(), (const BatchProblem *)(base + o_prob),
-                                   (const int4 *)(base + o_item), nitems, (int *)(base + o_cnt), grid, ctx->stream);
+      int lerr = numc_batch_launch(L.family, L.descs, L.prefix, L.owner, L.units, L.probs, L.items, L.nitems,
+                                   L.counter, L.grid, ctx->stream, nullptr);
       hipGraph_t graph = nullptr;
-      hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
       if (lerr) {
         if (graph) (void)hipGraphDestroy(graph);
-        cache->release();
+        e->release();
         return lerr;
       }
-      if (e != hipSuccess) {
-        cache->release();
-        return numc_hip_fail(e, "hipStreamEndCapture");
+      if (ce != hipSuccess) {
+        e->release();
+        return numc_hip_fail(ce, "hipStreamEndCapture");
       }
-      cache->graph = graph;
-      e = hipGraphInstantiate(&cache->exec, graph, nullptr, nullptr, 0);
-      if (e != hipSuccess) {
-        cache->release();
-        return numc_hip_fail(e, "hipGraphInstantiate");
+      e->graph = graph;
+      ce = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+      if (ce != hipSuccess) {
+        e->release();
+        return numc_hip_fail(ce, "hipGraphInstantiate");
       }
-      cache->key = key;
-      have_graph = true;
+      e->key = key;
+      entry = e;
     }
   }
 
-  if (have_graph) {
-    numc_prof_begin(ctx, PROF_BATCHED, family);
-    HIP_TRY(hipGraphLaunch(cache->exec, ctx->stream), "hipGraphLaunch");
-    numc_prof_end(ctx);
-    cache->replays++;
+  if (entry) {
+    entry->last_use = ++cache->tick;
+    if (ctx->prof) {
+      // profiling: the same kernels launched eagerly, each timed on the stream
+      const BatchLaunch &L = entry->launch;
+      if ((err = numc_batch_launch(L.family, L.descs, L.prefix, L.owner, L.units, L.probs, L.items, L.nitems,
+                                   L.counter, L.grid, ctx->stream, ctx)))
+        return err;
+    } else {
+      HIP_TRY(hipGraphLaunch(entry->exec, ctx->stream), "hipGraphLaunch");
+      cache->replays++;
+    }
   }
   for (size_t i = 0; i < count; i++) {
     const Prob &p = P[i];

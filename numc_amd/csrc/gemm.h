@@ -75,7 +75,7 @@ void numc_batch_set_threshold(int family, int t);
 int numc_batch_pack_blocks(int family, PackArgs &pk); // sets pk.mode
 int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, const int *owner,
                       long long units, const BatchProblem *probs, const int4 *items, int nitems, int *counter,
-                      int grid, hipStream_t st);
+                      int grid, hipStream_t st, NumcCtx *prof); // prof: time the kernels (eager launches only)
 
 const GemmConfig *numc_gemm_config(int family, int idx);
 // configuration for a strided batch of equal M x N slices

@@ -2171,20 +2171,24 @@ int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p) {
 
 int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, const int *owner,
                       long long units, const BatchProblem *probs, const int4 *items, int nitems, int *counter,
-                      int grid, hipStream_t st) {
+                      int grid, hipStream_t st, NumcCtx *prof) {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, 1LL, 0u);
   const unsigned nb = (unsigned)units; // blocks (numc_batch_pack_blocks summed)
   if (nb) {
+    if (prof) numc_prof_begin(prof, PROF_PACK, family);
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_F32>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
     else if (family == NUMC_FAMILY_SPLIT_BF16) hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_SPLIT_BF16>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
     else hipLaunchKernelGGL((k_pack_batched<NUMC_FAMILY_I8>), dim3(nb), dim3(256), 0, st, descs, prefix, owner, units);
+    if (prof) numc_prof_end(prof);
   }
+  if (prof) numc_prof_begin(prof, PROF_BATCHED, family);
   if (family == NUMC_FAMILY_F32)
     hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 16, 3, 64, 64, 32, 3, 3>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   else if (family == NUMC_FAMILY_SPLIT_BF16)
     hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4, 2>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   else
     hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 2, 64, 64, 64, 4, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  if (prof) numc_prof_end(prof);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "batched launch");
   return 0;

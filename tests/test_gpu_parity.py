@@ -511,6 +511,51 @@ def test_batched_many_uneven_and_graph_replay(ctx):
     assert nc.lib().numc_ctx_batch_graph_replays(ctx.h) - before == 3
 
 
+def test_batched_graph_cache_alternating_batches(ctx):
+    """two batches called in turn both stay cached (a one-entry cache would
+    re-plan and re-capture on every call); a fifth distinct batch evicts the
+    least recently used one; profiling relaunches the kernels eagerly"""
+    rng = np.random.default_rng(18)
+
+    def make(n):
+        As, Bs, Os, ref = [], [], [], []
+        for M, N, K in zip(_logu(rng, 1, 300, n), _logu(rng, 1, 300, n), _logu(rng, 1, 300, n)):
+            a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+            As.append(ctx.from_numpy(a))
+            Bs.append(ctx.from_numpy(b))
+            Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+            ref.append((a, b))
+        return As, Bs, Os, ref
+
+    batches = [make(n) for n in (5, 7, 3, 4, 6)]
+    L = nc.lib()
+    for As, Bs, Os, _ in batches[:2]:
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+    assert L.numc_ctx_batch_graphs_cached(ctx.h) == 2
+    before = L.numc_ctx_batch_graph_replays(ctx.h)
+    for rep in range(3):
+        for As, Bs, Os, ref in batches[:2]:
+            for O in Os:
+                O.write(np.zeros(O.shape, np.float32))
+            nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+            for (a, b), O in zip(ref, Os):
+                assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+    # every call of both batches after the first was a replay of its own graph
+    assert L.numc_ctx_batch_graph_replays(ctx.h) - before == 6
+    for As, Bs, Os, _ in batches[2:]:
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+    assert L.numc_ctx_batch_graphs_cached(ctx.h) == 4
+    # profiling: eager launches of the cached batch, each kernel timed
+    As, Bs, Os, ref = batches[4]
+    ctx.profile(True)
+    nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched (profiled)")
+    ms, n = ctx.profile_read(3, nc.FAMILY_F32)
+    ctx.profile(False)
+    assert n == 1 and ms > 0
+    for (a, b), O in zip(ref, Os):
+        assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+
+
 def test_batched_int8_widen_and_wrap(ctx):
     rng = np.random.default_rng(17)
     shapes = [(40, 300, 70), (128, 64, 128), (3, 5, 7), (260, 1000, 90)]
