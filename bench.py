@@ -320,18 +320,21 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
             As.append(A)
             Bs.append(B)
             Os.append(O)
-        # autotuned schedule: the 128x128-tile threshold measured on this batch
+        # autotuned schedule: the kernel (direct or packed) and the large-tile
+        # threshold, measured on this batch
         thresh = nc.matmul_batched_tune(ctx, As, Bs, Os, method, flags, 3)
         if thresh < 0:
             nc.check(thresh, "numc_matmul_batched_tune")
-        nc.check(nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), "batched warmup")  # builds the graph
+        direct, _ = nc.batched_schedule(fam)
+        H = nc.BatchHandles(As, Bs, Os)   # the handle arrays, built once as a C caller keeps them
+        nc.check(nc.matmul_batched(ctx, H, method=method, flags=flags), "batched warmup")  # builds the graph
         ctx.synchronize()
         ctx.mark(2)
         for _ in range(reps):
-            nc.check(nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), "batched")
+            nc.check(nc.matmul_batched(ctx, H, method=method, flags=flags), "batched")
         ctx.mark(3)
         t_batched = ctx.elapsed_ms(2, 3) / reps
-        t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul_batched(ctx, As, Bs, Os, method, -1, flags), fam,
+        t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul_batched(ctx, H, method=method, flags=flags), fam,
                                    max(2, steps // 2), 1, batched=True)
         # one call per problem, same stream, no graph
         for i in range(count):
@@ -344,14 +347,15 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
         t_sep = ctx.elapsed_ms(4, 5)
         nbytes = float(np.sum(ein * (M_ * K_ + K_ * N_) + eout * M_ * N_))
         e = {"ms_batched": round(t_batched, 4), "tflops_batched": round(flop / t_batched * 1e-9, 2),
-             "tile_threshold": thresh,
+             "tile_threshold": thresh, "kernel": "direct" if direct else "packed",
              "ms_separate_launches": round(t_sep, 3),
              "tflops_separate": round(flop / t_sep * 1e-9, 2),
              "speedup": round(t_sep / t_batched, 2),
              "kernel_ms": round(t["kernel_ms"], 4), "pack_ms": round(t["pack_ms"], 4),
              "pack_launches_per_call": t["pack_launches_per_call"],
              "roofline": roofline_entry(flop, nbytes, t["kernel_ms"], peak, unit,
-                                        "persistent batched GEMM kernel (avg of %d launches)" % max(2, steps // 2))}
+                                        "persistent batched GEMM kernel, %s operands (avg of %d launches)" %
+                                        ("direct" if direct else "packed", max(2, steps // 2)))}
         if with_cpu:
             r, why = cpu[0 if dt == nc.INT8 else 8]
             if r:

@@ -83,13 +83,19 @@ NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, size_t M, siz
                            const void *a, size_t rsa, size_t csa, const void *b, size_t rsb, size_t csb,
                            void *c, size_t rso, size_t cso, const NumcMatmulOpts *opts);
 
-/* Autotune the batched schedule for this batch: times the whole batch with
- * each candidate threshold for 128x128 tiles (a problem takes them when M and
- * N both exceed it, else 64x64) and keeps the fastest for the batch's dtype
- * family in later numc_matmul_batched calls.  Returns the threshold. */
+/* Autotune the batched schedule for this batch: times the whole batch under
+ * each candidate schedule and keeps the fastest for the batch's dtype family
+ * in later numc_matmul_batched calls.  A schedule is the kernel -- direct
+ * (float32 and int8 inputs: operands read where they lie, no pack pass;
+ * 64x64 / 32x32 tiles whose waves split K) or packed (one pack launch, then
+ * 128x128 / 64x64 tiles) -- and the threshold above which a problem's M and N
+ * give it the larger tile.  Returns the threshold. */
 NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, NumcArray *const *a,
                                       NumcArray *const *b, NumcArray *const *out,
                                       const NumcMatmulOpts *opts, int reps);
+/* the family's current batched schedule (direct 1 / packed 0, threshold) */
+NUMC_API int numc_matmul_batched_schedule(int family, int *direct, int *threshold);
+NUMC_API int numc_matmul_batched_set_schedule(int family, int direct, int threshold);
 
 /* number of times the context replayed one of its cached batched graphs */
 NUMC_API long long numc_ctx_batch_graph_replays(NumcCtx *ctx);
@@ -159,9 +165,13 @@ typedef enum {
 } NumcFamily;
 
 typedef struct {
-  int bm, bn, kc;     /* block tile rows, cols, k-chunk */
+  int bm, bn, kc;     /* block tile rows, cols, k-chunk (direct: k per MFMA fragment) */
   int waves_m, waves_n;
-  int split_k;        /* number of K slices (1 = none) */
+  int split_k;        /* number of K slices summed by a second launch (1 = none) */
+  int waves_k;        /* waves of one block splitting K, summed in LDS (1 = none) */
+  int direct;         /* 1: one launch reads A and B in place (any strides and
+                         alignment, no pack and no workspace), numc's gemmsup
+                         counterpart; 0: operands packed into fragment order */
 } NumcKernelConfig;
 
 NUMC_API int numc_kernel_config_count(int family);
@@ -170,11 +180,14 @@ NUMC_API int numc_kernel_config_get(int family, int index, NumcKernelConfig *out
  * numc_autotune_shape times every configuration of `family` on device scratch
  * operands of shape M x K @ K x N (flags: 1 A stored transposed, 2 B stored
  * transposed, 4 padded rows), whole calls, mean of `reps`, into ms_out.
- * The selector predicts log2(ms) per configuration with an MLP cost model and
- * numc_matmul runs the argmin; without a trained model (or when it was
- * trained for other configuration tables) an analytic model is used.  The
- * model covers families 0-4; the bfloat16/float16 families use exact-shape
- * pins, else the analytic model. */
+ * The selector predicts log2(ms) per configuration with an MLP cost model
+ * (every family); numc_matmul runs the argmin, except that where the model
+ * cannot separate its best few configurations it times them once on the call
+ * and keeps the winner for that exact shape (numc_selector_candidates;
+ * environment NUMC_AUTOTUNE=off disables this).  Without a trained model (or
+ * one trained for other configuration tables) an analytic model is used:
+ * numc's gemmsup rule, direct configurations below measured K and M*N
+ * thresholds, else a packed one by estimated rounds of work. */
 NUMC_API int numc_autotune_shape(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps,
                                  float *ms_out, int ms_len);
 NUMC_API int numc_autotune_release(NumcCtx *ctx);
@@ -189,6 +202,15 @@ NUMC_API int numc_selector_predict(int family, int M, int N, int K, int flags, f
 NUMC_API int numc_selector_tune(NumcCtx *ctx, int family, int M, int N, int K, int flags, int reps);
 /* pin (cfg >= 0) or unpin (cfg < 0) the configuration of an exact shape */
 NUMC_API int numc_selector_pin(int family, int M, int N, int K, int flags, int cfg);
+/* The configurations a numc_matmul call would consider for this shape: 1 when
+ * the choice is settled (an exact-shape entry, or a confident model), up to 3
+ * whose predictions lie within 2^0.1 of the best, or all of them when the best
+ * predicted call takes under 2^-5 ms; in predicted order into out (at most
+ * max).  Returns the count, 0 without a trained model.  allow_direct = 0
+ * leaves out the direct configurations (integers of more than one digit
+ * plane). */
+NUMC_API int numc_selector_candidates(int family, int M, int N, int K, int flags, int allow_direct, int *out,
+                                      int max);
 /* the analytic model used without a trained selector */
 NUMC_API int numc_selector_heuristic(int family, int M, int N, int K);
 /* the configuration numc_matmul would run for this shape: exact-shape table,

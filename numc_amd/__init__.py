@@ -68,7 +68,7 @@ class MatmulOpts(C.Structure):
 
 class KernelConfig(C.Structure):
     _fields_ = [("bm", C.c_int), ("bn", C.c_int), ("kc", C.c_int), ("waves_m", C.c_int),
-                ("waves_n", C.c_int), ("split_k", C.c_int)]
+                ("waves_n", C.c_int), ("split_k", C.c_int), ("waves_k", C.c_int), ("direct", C.c_int)]
 
 
 _P = C.c_void_p
@@ -121,11 +121,14 @@ _SIGS = {
                                            C.POINTER(MatmulOpts), C.c_int]),
     "numc_ctx_batch_graph_replays": (C.c_longlong, [_P]),
     "numc_ctx_batch_graphs_cached": (C.c_int, [_P]),
+    "numc_matmul_batched_schedule": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "numc_matmul_batched_set_schedule": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "numc_autotune_release": (C.c_int, [_P]),
     "numc_device_copy": (C.c_int, [_P, _P, _P, _SZ]),
     "numc_selector_tune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "numc_selector_pin": (C.c_int, [C.c_int] * 6),
     "numc_selector_active": (C.c_int, []),
+    "numc_selector_candidates": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.c_int]),
     "numc_array_to_device": (C.c_int, [_P]),
     "numc_array_to_host": (C.c_int, [_P]),
     "numc_array_on_device": (C.c_int, [_P]),
@@ -365,13 +368,30 @@ def matmul_batched_tune(ctx, As, Bs, Outs, method=METHOD_AUTO, flags=0, reps=3):
     return lib().numc_matmul_batched_tune(ctx.h, n, pa, pb, po, C.byref(opts), reps)
 
 
-def matmul_batched(ctx, As, Bs, Outs, method=METHOD_AUTO, config=-1, flags=0):
-    n = len(As)
-    pa = (C.c_void_p * n)(*[x.h for x in As])
-    pb = (C.c_void_p * n)(*[x.h for x in Bs])
-    po = (C.c_void_p * n)(*[x.h for x in Outs])
+def batched_schedule(family):
+    """(direct, threshold) of the family's batched schedule"""
+    d, t = C.c_int(), C.c_int()
+    check(lib().numc_matmul_batched_schedule(family, C.byref(d), C.byref(t)), "numc_matmul_batched_schedule")
+    return d.value, t.value
+
+
+class BatchHandles:
+    """the three handle arrays numc_matmul_batched takes, built once (a C caller
+    keeps its arrays of NumcArray pointers the same way)"""
+
+    def __init__(self, As, Bs, Outs):
+        self.n = len(As)
+        self.pa = (C.c_void_p * self.n)(*[x.h for x in As])
+        self.pb = (C.c_void_p * self.n)(*[x.h for x in Bs])
+        self.po = (C.c_void_p * self.n)(*[x.h for x in Outs])
+        self.keep = (As, Bs, Outs)
+
+
+def matmul_batched(ctx, As, Bs=None, Outs=None, method=METHOD_AUTO, config=-1, flags=0):
+    """numc_matmul_batched over lists of arrays, or over a BatchHandles (As)"""
+    h = As if isinstance(As, BatchHandles) else BatchHandles(As, Bs, Outs)
     opts = MatmulOpts(method, config, flags)
-    return lib().numc_matmul_batched(ctx.h, n, pa, pb, po, C.byref(opts))
+    return lib().numc_matmul_batched(ctx.h, h.n, h.pa, h.pb, h.po, C.byref(opts))
 
 
 def config_count(family):
@@ -381,4 +401,5 @@ def config_count(family):
 def config_get(family, idx):
     c = KernelConfig()
     check(lib().numc_kernel_config_get(family, idx, C.byref(c)), "numc_kernel_config_get")
-    return dict(bm=c.bm, bn=c.bn, kc=c.kc, waves_m=c.waves_m, waves_n=c.waves_n, split_k=c.split_k)
+    return dict(bm=c.bm, bn=c.bn, kc=c.kc, waves_m=c.waves_m, waves_n=c.waves_n, split_k=c.split_k,
+                waves_k=c.waves_k, direct=c.direct)

@@ -7,7 +7,7 @@
 sweep: shapes 0..19999 (training) and 20000..21999 (held out), drawn from a
 seeded generator: M, N log-uniform integers in [1, 16384] (15% skinny,
 M log-uniform in [1, 32]), K log-uniform in [1, 65536], a kernel family
-uniform over the five families, and layout flags (20% A stored transposed,
+uniform over the seven families, and layout flags (20% A stored transposed,
 20% B stored transposed, 10% padded rows).  Each shape's every configuration
 is timed by numc_autotune_shape (whole calls on device scratch operands).
 
@@ -30,6 +30,8 @@ import numpy as np
 from . import (FAMILY_NAMES, Ctx, check, config_count, config_get, lib)
 
 N_TRAIN, N_HOLDOUT = 20000, 2000
+N_FAMILIES = 7   # every kernel family (FAMILY_F32 .. FAMILY_F16)
+MODEL_VERSION = 2
 HIDDEN = 64
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_DB = os.path.join(HERE, "data", "autotune_db.csv.gz")
@@ -43,7 +45,7 @@ def _logu(rng, lo, hi, n):
 def sample_shapes(seed=1, n=N_TRAIN + N_HOLDOUT):
     """(family, M, N, K, flags) for every shape index, deterministic in seed."""
     rng = np.random.default_rng(seed)
-    fam = rng.integers(0, 5, n)
+    fam = rng.integers(0, N_FAMILIES, n)
     M = _logu(rng, 1, 16384, n)
     N = _logu(rng, 1, 16384, n)
     K = _logu(rng, 1, 65536, n)
@@ -66,6 +68,8 @@ def _setup():
     L.numc_selector_heuristic.restype = C.c_int
     L.numc_selector_choice.argtypes = [C.c_int] * 5
     L.numc_selector_choice.restype = C.c_int
+    L.numc_selector_candidates.argtypes = [C.c_int] * 6 + [C.POINTER(C.c_int), C.c_int]
+    L.numc_selector_candidates.restype = C.c_int
     L.numc_mlp_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64]
     L.numc_mlp_create.restype = C.c_void_p
     L.numc_mlp_free.argtypes = [C.c_void_p]
@@ -160,13 +164,14 @@ def design(db, idxs):
 def save_model(path, params, din, hidden):
     with open(path, "wb") as f:
         f.write(b"NMLS")
-        f.write(struct.pack("<5i", 1, din, hidden, 1, 5))
-        for fam in range(5):
+        f.write(struct.pack("<5i", MODEL_VERSION, din, hidden, 1, N_FAMILIES))
+        for fam in range(N_FAMILIES):
             n = config_count(fam)
             f.write(struct.pack("<i", n))
             for i in range(n):
                 c = config_get(fam, i)
-                f.write(struct.pack("<6i", c["bm"], c["bn"], c["kc"], c["waves_m"], c["waves_n"], c["split_k"]))
+                f.write(struct.pack("<8i", c["bm"], c["bn"], c["kc"], c["waves_m"], c["waves_n"], c["split_k"],
+                                    c["waves_k"], c["direct"]))
         f.write(np.asarray(params, np.float32).tobytes())
 
 
@@ -178,7 +183,7 @@ def load_model_params(path):
     o = 24
     for _ in range(nfam):
         (n,) = struct.unpack("<i", data[o:o + 4])
-        o += 4 + 24 * n
+        o += 4 + 32 * n
     return np.frombuffer(data[o:], np.float32).copy(), din, hidden, dout
 
 
@@ -216,7 +221,9 @@ def evaluate(db_path, model_path, idxs=None, use_gpu=False):
         idxs = [i for i in sorted(db) if i >= N_TRAIN]
     check(L.numc_selector_load(model_path.encode()), "numc_selector_load")
     pred_buf = (C.c_float * 16)()
-    reg_mlp, reg_heur, t_best, t_mlp, t_heur = [], [], 0.0, 0.0, 0.0
+    reg_mlp, reg_heur, reg_fb, t_best, t_mlp, t_heur, t_fb = [], [], [], 0.0, 0.0, 0.0, 0.0
+    n_timed = 0
+    cand = (C.c_int * 32)()
     by_family = {}
     for i in idxs:
         e = db[i]
@@ -226,14 +233,25 @@ def evaluate(db_path, model_path, idxs=None, use_gpu=False):
         check(L.numc_selector_predict(fam, M, N, K, fl, pred_buf, 16), "numc_selector_predict")
         choice = int(np.argmin([pred_buf[c] for c in range(config_count(fam))]))
         h = L.numc_selector_heuristic(fam, M, N, K)
+        # the call-time rule: where the model's best few are close (or the call
+        # is tiny), numc_matmul times them once and keeps the fastest
+        nc_ = L.numc_selector_candidates(fam, M, N, K, fl, 1, cand, 32)
+        fb = min((cand[i] for i in range(nc_)), key=lambda c: ms[c]) if nc_ > 0 else choice
+        n_timed += nc_ > 1
         reg_mlp.append(ms[choice] / best - 1)
         reg_heur.append(ms[h] / best - 1)
+        reg_fb.append(ms[fb] / best - 1)
         t_best += best
         t_mlp += ms[choice]
         t_heur += ms[h]
+        t_fb += ms[fb]
         by_family.setdefault(FAMILY_NAMES[fam], []).append(ms[choice] / best - 1)
-    rm, rh = np.array(reg_mlp), np.array(reg_heur)
+    rm, rh, rf = np.array(reg_mlp), np.array(reg_heur), np.array(reg_fb)
     return {"shapes": len(idxs),
+            "mlp_with_call_timing": {"mean_regret": float(rf.mean()), "p95_regret": float(np.percentile(rf, 95)),
+                                     "max_regret": float(rf.max()), "total_time_ratio": t_fb / t_best,
+                                     "exact_best_frac": float(np.mean(rf == 0)),
+                                     "shapes_timed": int(n_timed)},
             "mlp": {"mean_regret": float(rm.mean()), "p95_regret": float(np.percentile(rm, 95)),
                     "max_regret": float(rm.max()), "total_time_ratio": t_mlp / t_best,
                     "exact_best_frac": float(np.mean(rm == 0))},

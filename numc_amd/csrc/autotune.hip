@@ -149,3 +149,47 @@ extern "C" NUMC_API int numc_autotune_shape(NumcCtx *ctx, int family, int M, int
   }
   return 0;
 }
+
+// Time candidate configurations on one call's own operands and return the
+// fastest in *best: each candidate runs once to warm up, then once timed (five
+// times when a run takes under 50 us, where launch noise is of the order of
+// the differences).  The output is rewritten by every run; the caller's run of
+// the winner writes the result.  The runs stay out of the kernel profile.
+int numc_tune_call(NumcCtx *ctx, const GemmProblem &p, const int *cand, int n, int *best) {
+  TuneBuffers *tb = tune_buffers(ctx);
+  if (!tb->e0) {
+    HIP_TRY(hipEventCreate(&tb->e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&tb->e1), "hipEventCreate");
+  }
+  void *prof = ctx->prof;
+  ctx->prof = nullptr;
+  int err = 0;
+  float best_ms = 0.f;
+  *best = cand[0];
+  for (int i = 0; i < n && !err; i++) {
+    const GemmConfig *c = numc_gemm_config(p.family, cand[i]);
+    if (!c) continue;
+    size_t aux = 0, need = numc_gemm_workspace(p, *c, &aux);
+    if ((err = numc_ws_reserve(ctx, &ctx->ws, need)) || (err = numc_ws_reserve(ctx, &ctx->ws_aux, aux))) break;
+    if ((err = numc_gemm_launch(ctx, p, *c, ctx->ws.ptr, ctx->ws_aux.ptr))) break;
+    float ms = 0.f;
+    for (int reps = 1; reps <= 5 && !err; reps += 4) {
+      if (hipEventRecord(tb->e0, ctx->stream) != hipSuccess) err = NUMC_ERR_INTERNAL;
+      for (int r = 0; r < reps && !err; r++) err = numc_gemm_launch(ctx, p, *c, ctx->ws.ptr, ctx->ws_aux.ptr);
+      if (err) break;
+      if (hipEventRecord(tb->e1, ctx->stream) != hipSuccess || hipEventSynchronize(tb->e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, tb->e0, tb->e1) != hipSuccess) {
+        err = NUMC_SET_ERROR(NUMC_ERR_INTERNAL, "autotune: event timing failed");
+        break;
+      }
+      ms /= (float)reps;
+      if (ms >= 0.05f) break;
+    }
+    if (!err && (i == 0 || ms < best_ms)) {
+      best_ms = ms;
+      *best = cand[i];
+    }
+  }
+  ctx->prof = prof;
+  return err;
+}

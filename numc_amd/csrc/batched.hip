@@ -42,6 +42,8 @@ struct BatchLaunch {
   int nitems = 0;
   int *counter = nullptr;
   int grid = 0;
+  bool direct = false;               // the direct kernel on the operands as they lie
+  const BatchDirect *dprobs = nullptr;
 };
 
 struct BatchEntry {
@@ -167,6 +169,13 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     int nRTa, nRTb, nKF;
   };
   const int thresh = numc_batch_threshold(family);
+  // the direct kernel reads one int8 digit plane: int8 inputs only
+  bool direct = numc_batch_direct(family) &&
+                (family == NUMC_FAMILY_F32 || (family == NUMC_FAMILY_I8 && same_in && in_dt == NUMC_DTYPE_INT8));
+  // int8: only when every A row and every column of B is k-contiguous (B
+  // stored transposed); other int8 batches run the packed kernel
+  for (size_t i = 0; direct && family == NUMC_FAMILY_I8 && i < count; i++)
+    direct = a[i]->strides[1] == ein && b[i]->strides[0] == ein;
   std::vector<Prob> P(count);
   for (size_t i = 0; i < count; i++) {
     Prob &p = P[i];
@@ -197,7 +206,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   std::vector<uint64_t> key;
   key.reserve(4 + count * 12);
   key.push_back((uint64_t)family);
-  key.push_back((uint64_t)(uint32_t)thresh);
+  key.push_back((uint64_t)(uint32_t)thresh | ((uint64_t)direct << 32));
   key.push_back((uint64_t)out_dt);
   key.push_back(count);
   for (size_t i = 0; i < count; i++) {
@@ -211,7 +220,94 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   BatchCache *cache = (BatchCache *)ctx->batch_cache;
 
   BatchEntry *entry = cache->find(key);
-  if (!entry) {
+  if (!entry && direct) {
+    // ---- plan of the direct kernel: tile class per problem, work items
+    // longest first; the tables are all it needs (no packed operands) ----
+    std::vector<BatchDirect> dp(count);
+    std::vector<std::pair<double, int4>> work;
+    for (size_t i = 0; i < count; i++) {
+      const Prob &p = P[i];
+      BatchDirect &d = dp[i];
+      memset(&d, 0, sizeof d);
+      if (!p.C) continue;
+      d.A = (const unsigned char *)p.A;
+      d.a_rs = p.rsa * (long long)ein;
+      d.a_ks = p.csa * (long long)ein;
+      d.B = (const unsigned char *)p.B;
+      d.b_rs = p.csb * (long long)ein; // rows of B^T
+      d.b_ks = p.rsb * (long long)ein;
+      d.M = p.M;
+      d.N = p.N;
+      d.K = p.K;
+      d.C = p.C;
+      d.rso = p.rso;
+      d.cso = p.cso;
+      d.out_bytes = (int)eout;
+      const int cls = (p.M > thresh && p.N > thresh) ? 0 : 1, t = cls == 0 ? 64 : 32;
+      // a tile's time: its multiply-adds plus a fixed start-up (one wave per
+      // SIMD streams K / 4 of them), so long-K tiles go first
+      const double tile_time = (double)t * t * (p.K > 0 ? p.K : 1) + 2.0e5;
+      const int nbm = (int)cdivll(p.M, t), nbn = (int)cdivll(p.N, t);
+      for (int x = 0; x < nbm; x++)
+        for (int y = 0; y < nbn; y++) work.push_back({tile_time, make_int4((int)i, x, y, cls)});
+    }
+    std::stable_sort(work.begin(), work.end(), [](const std::pair<double, int4> &x, const std::pair<double, int4> &y) {
+      return x.first > y.first;
+    });
+    if (work.size() > (size_t)INT_MAX)
+      return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched: too many tiles for one launch (%zu)", work.size());
+    const int nitems = (int)work.size();
+    if (nitems > 0) {
+      std::vector<int4> items(work.size());
+      for (size_t x = 0; x < work.size(); x++) items[x] = work[x].second;
+      const size_t o_prob = 0;
+      const size_t o_item = align_up(o_prob + dp.size() * sizeof(BatchDirect), 256);
+      const size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
+      const size_t total = o_cnt + 256;
+      BatchEntry *e = cache->victim();
+      HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      e->release();
+      HIP_TRY(hipMalloc(&e->dev, total), "hipMalloc(batched tables)");
+      e->dev_bytes = total;
+      unsigned char *base = (unsigned char *)e->dev;
+      HIP_TRY(hipMemcpyAsync(base + o_prob, dp.data(), dp.size() * sizeof(BatchDirect), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipMemcpyAsync(base + o_item, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      int cus = 256;
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+      BatchLaunch &L = e->launch;
+      L = BatchLaunch();
+      L.family = family;
+      L.direct = true;
+      L.dprobs = (const BatchDirect *)(base + o_prob);
+      L.items = (const int4 *)(base + o_item);
+      L.nitems = nitems;
+      L.counter = (int *)(base + o_cnt);
+      L.grid = std::max(1, std::min(nitems, numc_batch_direct_occupancy(family) * cus));
+      HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      int lerr = numc_batch_direct_launch(family, L.dprobs, L.items, L.nitems, L.counter, L.grid, ctx->stream, nullptr);
+      hipGraph_t graph = nullptr;
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+      if (lerr) {
+        if (graph) (void)hipGraphDestroy(graph);
+        e->release();
+        return lerr;
+      }
+      if (ce != hipSuccess) {
+        e->release();
+        return numc_hip_fail(ce, "hipStreamEndCapture");
+      }
+      e->graph = graph;
+      ce = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+      if (ce != hipSuccess) {
+        e->release();
+        return numc_hip_fail(ce, "hipGraphInstantiate");
+      }
+      e->key = key;
+      entry = e;
+    }
+  }
+  if (!entry && !direct) {
     // ---- plan: tile class and packed layout per problem ----
     long long packed = 0;
     for (size_t i = 0; i < count; i++) {
@@ -329,6 +425,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
       const int grid = std::max(1, std::min(nitems, numc_batch_occupancy(family) * cus));
       BatchLaunch &L = e->launch;
+      L = BatchLaunch();
       L.family = family;
       L.descs = (const PackArgs *)(base + o_desc);
       L.prefix = (const long long *)(base + o_pref);
@@ -369,9 +466,12 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
     if (ctx->prof) {
       // profiling: the same kernels launched eagerly, each timed on the stream
       const BatchLaunch &L = entry->launch;
-      if ((err = numc_batch_launch(L.family, L.descs, L.prefix, L.owner, L.units, L.probs, L.items, L.nitems,
-                                   L.counter, L.grid, ctx->stream, ctx)))
-        return err;
+      if (L.direct)
+        err = numc_batch_direct_launch(L.family, L.dprobs, L.items, L.nitems, L.counter, L.grid, ctx->stream, ctx);
+      else
+        err = numc_batch_launch(L.family, L.descs, L.prefix, L.owner, L.units, L.probs, L.items, L.nitems, L.counter,
+                                L.grid, ctx->stream, ctx);
+      if (err) return err;
     } else {
       HIP_TRY(hipGraphLaunch(entry->exec, ctx->stream), "hipGraphLaunch");
       cache->replays++;
@@ -386,9 +486,11 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   return numc_finish(ctx);
 }
 
-// Autotune the batch schedule: time the whole batch (graph replays) with the
-// 128x128/64x64 threshold at each candidate and keep the fastest for the
-// family.  Returns the chosen threshold.
+// Autotune the batch schedule: time the whole batch (graph replays) under each
+// candidate schedule -- the direct kernel (FP32 and int8 batches) or the
+// packed one, and the threshold above which a problem takes the larger tile
+// class -- and keep the fastest for the family.  Returns the chosen threshold
+// (numc_matmul_batched_schedule reports the kernel).
 extern "C" NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, NumcArray *const *a, NumcArray *const *b,
                                                  NumcArray *const *out, const NumcMatmulOpts *opts, int reps) {
   if (!ctx || !count || !a || !b || !out) return NUMC_SET_ERROR(NUMC_ERR_NULL, "matmul_batched_tune: NULL argument");
@@ -398,30 +500,55 @@ extern "C" NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, Num
   int err = numc_gpu_init(ctx);
   if (err) return err;
   static const int cand[] = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30};
+  const bool can_direct = family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
   HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
   int best = numc_batch_threshold(family);
+  bool best_direct = numc_batch_direct(family);
   float best_ms = 0.f;
-  for (int t : cand) {
-    numc_batch_set_threshold(family, t);
-    if ((err = numc_matmul_batched(ctx, count, a, b, out, opts))) break; // builds this schedule's graph
-    if ((err = (int)hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : NUMC_ERR_INTERNAL)) break;
-    (void)hipEventRecord(e0, ctx->stream);
-    for (int r = 0; r < std::max(1, reps); r++)
-      if ((err = numc_matmul_batched(ctx, count, a, b, out, opts))) break;
-    if (err) break;
-    (void)hipEventRecord(e1, ctx->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (best_ms == 0.f || ms < best_ms) {
-      best_ms = ms;
-      best = t;
+  for (int d = can_direct ? 1 : 0; d >= 0 && !err; d--) {
+    numc_batch_set_direct(family, d != 0);
+    for (int t : cand) {
+      numc_batch_set_threshold(family, t);
+      if ((err = numc_matmul_batched(ctx, count, a, b, out, opts))) break; // builds this schedule's graph
+      if ((err = (int)hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : NUMC_ERR_INTERNAL)) break;
+      (void)hipEventRecord(e0, ctx->stream);
+      for (int r = 0; r < std::max(1, reps); r++)
+        if ((err = numc_matmul_batched(ctx, count, a, b, out, opts))) break;
+      if (err) break;
+      (void)hipEventRecord(e1, ctx->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (best_ms == 0.f || ms < best_ms) {
+        best_ms = ms;
+        best = t;
+        best_direct = d != 0;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  numc_batch_set_direct(family, best_direct);
   numc_batch_set_threshold(family, best);
   return err ? err : best;
+}
+
+extern "C" NUMC_API int numc_matmul_batched_schedule(int family, int *direct, int *threshold) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT || !direct || !threshold)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched_schedule: bad argument");
+  *direct = numc_batch_direct(family) ? 1 : 0;
+  *threshold = numc_batch_threshold(family);
+  return 0;
+}
+
+extern "C" NUMC_API int numc_matmul_batched_set_schedule(int family, int direct, int threshold) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched_set_schedule: bad family %d", family);
+  if (direct && family != NUMC_FAMILY_F32 && family != NUMC_FAMILY_I8)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched_set_schedule: no direct batched kernel for family %d", family);
+  numc_batch_set_direct(family, direct != 0);
+  numc_batch_set_threshold(family, threshold);
+  return 0;
 }

@@ -37,6 +37,8 @@ struct GemmConfig {
   // same kernel reading operands in place (GemmArgs::Araw / Braw), or null:
   // [A, B, A and B][plain store, bias store]
   GemmLaunchFn inplace[3][2];
+  // direct configurations (cfg.direct): launch / launch_bias run the one-launch
+  // kernel on the raw operands (GemmArgs::Araw, a_rs, a_ks, Braw, b_rs, b_ks)
 };
 
 struct PackArgs {
@@ -63,6 +65,21 @@ struct BatchProblem {
   int out_bytes;
 };
 
+// one problem of the persistent direct batched kernel (operands in place;
+// B given as the rows of B^T: row j at B + j*b_rs, k stride b_ks; bytes)
+struct BatchDirect {
+  const unsigned char *A, *B;
+  long long a_rs, a_ks, b_rs, b_ks;
+  int M, N, K;
+  void *C;
+  long long rso, cso;
+  int out_bytes;
+};
+int numc_batch_direct_occupancy(int family);
+// direct batched kernel (FP32 and int8 families; gemm_small.hip): classes 0 = 64x64, 1 = 32x32 tiles
+int numc_batch_direct_launch(int family, const BatchDirect *probs, const int4 *items, int nitems, int *counter,
+                             int grid, hipStream_t st, NumcCtx *prof);
+
 int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc);
 int numc_batch_occupancy(int family);
 struct BatchChoice {
@@ -72,12 +89,21 @@ struct BatchChoice {
 BatchChoice numc_batch_choice(int family, int M, int N, int K, int thresh);
 int numc_batch_threshold(int family);
 void numc_batch_set_threshold(int family, int t);
+// whether the family's batches run the direct kernel (no pack) or the packed one
+bool numc_batch_direct(int family);
+void numc_batch_set_direct(int family, bool direct);
 int numc_batch_pack_blocks(int family, PackArgs &pk); // sets pk.mode
 int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, const int *owner,
                       long long units, const BatchProblem *probs, const int4 *items, int nitems, int *counter,
                       int grid, hipStream_t st, NumcCtx *prof); // prof: time the kernels (eager launches only)
 
 const GemmConfig *numc_gemm_config(int family, int idx);
+// the direct (in-place, one-launch) configurations of a family (gemm_small.hip);
+// numc_gemm_config lists them after the packed ones
+const GemmConfig *numc_direct_configs(int family, int *count);
+// whether a direct configuration can run problem p (one digit plane for
+// integers, no strided batch; FP8 has none)
+bool numc_direct_ok(const GemmProblem &p);
 // configuration for a strided batch of equal M x N slices
 const GemmConfig *numc_strided_batch_config(int family, int M, int N);
 int numc_gemm_config_count(int family);
@@ -99,5 +125,10 @@ struct ScaleArgs {
 };
 int numc_scale_launch(NumcCtx *ctx, const ScaleArgs &a);
 int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p);
-// heuristic / learned choice of config index for a family and shape
-int numc_select_config(int family, int M, int N, int K, int flags);
+// int8 digit planes of an integer problem (1 for the other families)
+int numc_problem_planes(const GemmProblem &p);
+// time the candidate configurations on the problem's own operands (autotune.hip)
+int numc_tune_call(NumcCtx *ctx, const GemmProblem &p, const int *cand, int n, int *best);
+// heuristic / learned choice of config index for a family and shape (direct
+// configurations only when allowed)
+int numc_select_config(int family, int M, int N, int K, int flags, bool allow_direct);

@@ -1071,11 +1071,11 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
 }
 
 #define CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, VAR) \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>, nullptr}
+  {{BM, BN, KC, WGM, WGN, SPLIT, 1, 0}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE, EPI, NS, VAR>, nullptr}
 #define CFGS(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS) CFGP(MF, BM, BN, KC, WGM, WGN, MODE, EPI, SPLIT, NS, 0)
 // plain FP32 / FP64 / integer configurations also compile a biased-store twin (EPI + 4)
 #define CFGB(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                      \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
+  {{BM, BN, KC, WGM, WGN, SPLIT, 1, 0}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>}
 
 // ... and also the twins that read A, B or both in place (FP32 and int8; the
@@ -1085,7 +1085,7 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
   {&launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, V>,                            \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, V>}
 #define CFGD(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                      \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
+  {{BM, BN, KC, WGM, WGN, SPLIT, 1, 0}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>,                                  \
    {CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 2), CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 4),             \
     CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 6)}}
@@ -1094,7 +1094,7 @@ static void launch_gemm(const GemmArgs &g, dim3 grid, hipStream_t s) {
 // 0.0951 ms with B packed) and an FP32 configuration whose B-in-place twins
 // would spill
 #define CFGDA(MF, BM, BN, KC, WGM, WGN, EPI, SPLIT, NS)                                                     \
-  {{BM, BN, KC, WGM, WGN, SPLIT}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
+  {{BM, BN, KC, WGM, WGN, SPLIT, 1, 0}, MF::T, &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI, NS, 0>, \
    &launch_gemm<MF, BM, BN, KC, WGM, WGN, MODE_SINGLE, EPI + 4, NS, 0>,                                  \
    {CFGIP(MF, BM, BN, KC, WGM, WGN, EPI, NS, 2), {nullptr, nullptr}, {nullptr, nullptr}}}
 
@@ -1206,15 +1206,21 @@ static const FamilyTable kFamilies[NUMC_FAMILY_COUNT] = {
     {kF16, (int)(sizeof kF16 / sizeof kF16[0])},
 };
 
+// a family's packed configurations, then its direct ones
 const GemmConfig *numc_gemm_config(int family, int idx) {
-  if (family < 0 || family >= NUMC_FAMILY_COUNT) return nullptr;
-  if (idx < 0 || idx >= kFamilies[family].n) return nullptr;
-  return &kFamilies[family].cfg[idx];
+  if (family < 0 || family >= NUMC_FAMILY_COUNT || idx < 0) return nullptr;
+  if (idx < kFamilies[family].n) return &kFamilies[family].cfg[idx];
+  int nd = 0;
+  const GemmConfig *d = numc_direct_configs(family, &nd);
+  idx -= kFamilies[family].n;
+  return idx < nd ? &d[idx] : nullptr;
 }
 
 int numc_gemm_config_count(int family) {
   if (family < 0 || family >= NUMC_FAMILY_COUNT) return 0;
-  return kFamilies[family].n;
+  int nd = 0;
+  numc_direct_configs(family, &nd);
+  return kFamilies[family].n + nd;
 }
 
 extern "C" NUMC_API int numc_kernel_config_count(int family) { return numc_gemm_config_count(family); }
@@ -1879,7 +1885,20 @@ static void launch_rowmax(const float *src, long long rs, long long ks, int R, i
   }
 }
 
+int numc_problem_planes(const GemmProblem &p) {
+  return p.family == NUMC_FAMILY_I8 ? digit_planes(p.in_dt, p.out_dt) : 1;
+}
+
+bool numc_direct_ok(const GemmProblem &p) {
+  if (p.batch > 1 || p.family == NUMC_FAMILY_FP8) return false;
+  return p.family != NUMC_FAMILY_I8 || digit_planes(p.in_dt, p.out_dt) == 1;
+}
+
 size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *aux_bytes) {
+  if (c.cfg.direct) { // operands read in place: no workspace
+    if (aux_bytes) *aux_bytes = 0;
+    return 0;
+  }
   int kf = family_kf(p.family), nRT, nKF;
   long long pa, pb;
   plane_geometry(c, p.M, p.K, kf, c.cfg.bm, &nRT, &nKF, &pa);
@@ -1908,6 +1927,42 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     for (int z = 0; z < (p.batch > 1 ? p.batch : 1); z++)
       hipLaunchKernelGGL(k_zero_strided, dim3(blocks_for(n)), dim3(256), 0, st, (unsigned char *)p.C + z * p.c_bs * es,
                          p.M, p.N, p.rso, p.cso, es, (const unsigned char *)p.bias, p.bias_s);
+    return 0;
+  }
+
+  if (c.cfg.direct) {
+    // one launch on the operands as they lie (gemm_small.hip)
+    if (!numc_direct_ok(p))
+      return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: a direct configuration needs one digit plane and no batch");
+    const size_t ein = numc_dtype_size(p.in_dt);
+    GemmArgs g{};
+    g.Araw = (const unsigned char *)p.A;
+    g.a_rs = p.rsa * (long long)ein;
+    g.a_ks = p.csa * (long long)ein;
+    g.Braw = (const unsigned char *)p.B; // rows of B^T: column j at B + j*csb, k stride rsb
+    g.b_rs = p.csb * (long long)ein;
+    g.b_ks = p.rsb * (long long)ein;
+    g.M = p.M;
+    g.N = p.N;
+    g.K = p.K;
+    g.nbm = (int)cdiv(p.M, c.cfg.bm);
+    g.nbn = (int)cdiv(p.N, c.cfg.bn);
+    g.ksplit = 1;
+    g.C = p.C;
+    g.rso = p.rso;
+    g.cso = p.cso;
+    g.out_bytes = (int)numc_dtype_size(p.out_dt);
+    g.bias = p.bias;
+    g.bias_s = p.bias_s;
+    g.nprod = 1;
+    if ((long long)g.nbm * g.nbn > 0x7FFFFFFFLL) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: too many tiles");
+    GemmLaunchFn fn = p.bias ? c.launch_bias : c.launch;
+    if (!fn) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: a bias needs the plain method (family %d)", family);
+    numc_prof_begin(ctx, PROF_GEMM, family);
+    fn(g, dim3((unsigned)((long long)g.nbm * g.nbn)), st);
+    numc_prof_end(ctx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return numc_hip_fail(e, "direct gemm launch");
     return 0;
   }
 

@@ -166,6 +166,8 @@ struct GemmArgs {
   long long a_rs;             // a_rs bytes apart, 16-byte aligned, K a multiple of the k-chunk
   const unsigned char *Braw;  // B read in place (VAR & 4): the same for the rows of B^T
   long long b_rs;
+  long long a_ks, b_ks;       // direct kernels: bytes between consecutive k of A / of B^T (any value)
+  int K;                      // direct kernels: the inner dimension
 };
 
 // Nonzero-digit-plane masks of the integer packs: each operand's mask is

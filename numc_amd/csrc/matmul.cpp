@@ -10,6 +10,7 @@
 #include "internal.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -105,21 +106,59 @@ static int stage_out_end(NumcCtx *ctx, const NumcArray *x, void *d) {
 
 // ---- config choice ------------------------------------------------------------
 
-int numc_selector_choose(int family, int M, int N, int K, int flags); // selector.cpp
+int numc_selector_choose(int family, int M, int N, int K, int flags, bool allow_direct); // selector.cpp
+void numc_selector_record(int family, int M, int N, int K, int flags, int cfg);
+extern "C" NUMC_API int numc_selector_candidates(int family, int M, int N, int K, int flags, int allow_direct, int *out,
+                                                 int max);
 
-// Analytic default: estimated time = rounds of 256 CUs x per-block work at a
-// tile-dependent efficiency, plus the split-K reduction traffic.
-static int heuristic_config(int family, int M, int N, int K) {
+// timing the model's close candidates on a call (once per exact shape); the
+// environment variable NUMC_AUTOTUNE=off keeps to the model's best guess
+static bool call_tuning_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("NUMC_AUTOTUNE");
+    return !(e && strcmp(e, "off") == 0);
+  }();
+  return on;
+}
+
+// Analytic default (used without a trained cost model).  Direct
+// configurations where K <= 2^kt and M*N <= 2^mt for the family -- numc's
+// gemmsup rule (src/matmul/matmul.c:241-256) with thresholds measured on this
+// GPU: the values minimise the mean regret over the 20000 training shapes of
+// the autotuner's database (numc_amd/data/autotune_db.csv.gz) -- with 64x64
+// tiles when they alone make 256 tiles, else 32x32.  Otherwise a packed
+// configuration by rounds of 256 CUs x per-block work at a tile-dependent
+// efficiency, plus the split-K reduction.
+static const int kDirectLog2K[NUMC_FAMILY_COUNT] = {11, 8, -1, 11, 11, 11, 11};
+static const int kDirectLog2MN[NUMC_FAMILY_COUNT] = {22, 20, -1, 20, 22, 20, 20};
+
+static int heuristic_config(int family, int M, int N, int K, bool allow_direct) {
+  if (allow_direct && kDirectLog2K[family] >= 0 && (long long)K <= (1LL << kDirectLog2K[family]) &&
+      (long long)M * N <= (1LL << kDirectLog2MN[family])) {
+    int small = -1, big = -1;
+    for (int i = 0; i < numc_gemm_config_count(family); i++) {
+      const GemmConfig *c = numc_gemm_config(family, i);
+      if (!c->cfg.direct) continue;
+      if (small < 0 || c->cfg.bm < numc_gemm_config(family, small)->cfg.bm) small = i;
+      if (big < 0 || c->cfg.bm > numc_gemm_config(family, big)->cfg.bm) big = i;
+    }
+    if (big >= 0) {
+      const int bm = numc_gemm_config(family, big)->cfg.bm;
+      const long long tiles = ((long long)M + bm - 1) / bm * (((long long)N + bm - 1) / bm);
+      return tiles >= 256 ? big : small;
+    }
+  }
   int best = 0;
   double best_t = 1e300;
   for (int i = 0; i < numc_gemm_config_count(family); i++) {
     const GemmConfig *c = numc_gemm_config(family, i);
-    double nbm = std::ceil((double)M / c->cfg.bm), nbn = std::ceil((double)N / c->cfg.bn);
-    double blocks = nbm * nbn * c->cfg.split_k;
-    double rounds = std::ceil(blocks / 256.0);
-    double area = (double)c->cfg.bm * c->cfg.bn;
-    double eff = area >= 65536 ? 1.0 : area >= 32768 ? 0.9 : area >= 16384 ? 0.8 : 0.45;
-    double kb = std::ceil((double)K / c->cfg.split_k / c->cfg.kc) * c->cfg.kc;
+    if (c->cfg.direct) continue;
+    const double nbm = std::ceil((double)M / c->cfg.bm), nbn = std::ceil((double)N / c->cfg.bn);
+    const double blocks = nbm * nbn * c->cfg.split_k;
+    const double rounds = std::ceil(blocks / 256.0);
+    const double area = (double)c->cfg.bm * c->cfg.bn;
+    const double eff = area >= 65536 ? 1.0 : area >= 32768 ? 0.9 : area >= 16384 ? 0.8 : 0.45;
+    const double kb = std::ceil((double)K / c->cfg.split_k / c->cfg.kc) * c->cfg.kc;
     double t = rounds * area * kb / eff + 2000.0 * rounds;
     if (c->cfg.split_k > 1) t += (double)M * N * c->cfg.split_k * 0.5 + 20000.0;
     if (t < best_t) {
@@ -132,18 +171,18 @@ static int heuristic_config(int family, int M, int N, int K) {
 
 extern "C" NUMC_API int numc_selector_heuristic(int family, int M, int N, int K) {
   if (family < 0 || family >= NUMC_FAMILY_COUNT) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "bad family %d", family);
-  return heuristic_config(family, M, N, K);
+  return heuristic_config(family, M, N, K, true);
 }
 
-int numc_select_config(int family, int M, int N, int K, int flags) {
-  int s = numc_selector_choose(family, M, N, K, flags);
+int numc_select_config(int family, int M, int N, int K, int flags, bool allow_direct) {
+  int s = numc_selector_choose(family, M, N, K, flags, allow_direct);
   if (s >= 0 && s < numc_gemm_config_count(family)) return s;
-  return heuristic_config(family, M, N, K);
+  return heuristic_config(family, M, N, K, allow_direct);
 }
 
 extern "C" NUMC_API int numc_selector_choice(int family, int M, int N, int K, int flags) {
   if (family < 0 || family >= NUMC_FAMILY_COUNT) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "bad family %d", family);
-  return numc_select_config(family, M, N, K, flags);
+  return numc_select_config(family, M, N, K, flags, true);
 }
 
 // ---- entry points ---------------------------------------------------------------
@@ -157,9 +196,35 @@ static int launch_problem(NumcCtx *ctx, const GemmProblem &p, const NumcMatmulOp
   // layout flags as the autotuner measures them: A stored K x M, B stored N x K, padded rows
   const int lay = (p.csa != 1 ? 1 : 0) | (p.csb != 1 ? 2 : 0) |
                   ((p.csa == 1 && p.rsa > p.K) || (p.csb == 1 && p.rsb > p.N) ? 4 : 0);
-  int cfg_idx = opts && opts->config >= 0 ? opts->config : numc_select_config(p.family, p.M, p.N, p.K, lay);
+  const bool direct_ok = numc_direct_ok(p);
+  int cfg_idx;
+  if (opts && opts->config >= 0) {
+    cfg_idx = opts->config;
+  } else {
+    // the learned choice; where the model cannot separate its best few
+    // configurations, time them once on this call and keep the winner for the
+    // shape (the exhaustive autotuner's rule, on the candidates only).  An
+    // integer problem's digit-plane count is part of the shape's entry.
+    const int planes = numc_problem_planes(p);
+    const int key = lay | (planes > 1 ? planes << 4 : 0);
+    int cand[32];
+    const int nc = numc_selector_candidates(p.family, p.M, p.N, p.K, key, direct_ok, cand, 32);
+    if (nc > 1 && call_tuning_enabled()) {
+      int err = numc_tune_call(ctx, p, cand, nc, &cfg_idx);
+      if (err) return err;
+      numc_selector_record(p.family, p.M, p.N, p.K, key, cfg_idx);
+    } else if (nc >= 1) {
+      cfg_idx = cand[0];
+    } else {
+      cfg_idx = heuristic_config(p.family, p.M, p.N, p.K, direct_ok);
+    }
+  }
   const GemmConfig *cfg = numc_gemm_config(p.family, cfg_idx);
   if (!cfg) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul: no config %d for family %d", cfg_idx, p.family);
+  if (cfg->cfg.direct && !direct_ok)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE,
+                          "matmul: config %d reads the operands in place, which needs one int8 digit plane "
+                          "(int8 inputs, or a one-byte result)", cfg_idx);
   size_t aux = 0, need = numc_gemm_workspace(p, *cfg, &aux);
   int err;
   if ((err = numc_ws_reserve(ctx, &ctx->ws, need))) return err;

@@ -5,10 +5,13 @@
 // tables.  numc's counterpart is its compile-time threshold table
 // (src/matmul/matmul.c:241-256) choosing gemmsup vs packed.
 //
-// Model input (must match numc_amd/autotune.py `features`):
-//   log2 M/16, log2 N/16, log2 K/16, family one-hot (5), transposed A,
+// Model input (numc_selector_features; numc_amd/autotune.py builds its
+// training rows with it):
+//   log2 M/16, log2 N/16, log2 K/16, family one-hot (7), transposed A,
 //   transposed B, padded rows, log2 bm/8, log2 bn/8, log2 kc/8,
-//   log2(waves)/3, log2(split)/3, log2(blocks)/16, log2(rounds of 256)/8
+//   log2(waves)/3, log2(split)/3, log2(blocks)/16, log2(rounds of 256)/8,
+//   log2(waves splitting K)/2, direct (operands read in place by one launch),
+//   A read in place by the packed kernel, B read in place by the packed kernel
 // Model output: log2 of the predicted call time in ms.
 #include "gemm.h"
 #include "internal.h"
@@ -18,16 +21,17 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 namespace {
-constexpr int kDin = 18;
-// families the cost model covers (f32, split-bf16, fp8, f64, int8 digits);
-// the 16-bit float families use the exact-shape table, else the analytic model
-constexpr int kFam = 5;
+constexpr int kDin = 24;
+// families the cost model covers: all of them
+constexpr int kFam = NUMC_FAMILY_COUNT;
+constexpr int kFileVersion = 2;
 
 struct HostMlp {
   int din = 0, H = 0, dout = 0;
@@ -85,7 +89,8 @@ std::string default_path() {
   return "numc_amd/data/selector.bin";
 }
 
-// file: "NMLS", version 1, din, hidden, dout, nfam, per family {ncfg, ncfg x 6 ints}, float params
+// file: "NMLS", version 2, din, hidden, dout, nfam, per family {ncfg, ncfg x 8 ints
+// (NumcKernelConfig)}, float params
 bool load_file(const char *path, HostMlp *m, char *why, size_t whylen) {
   FILE *f = fopen(path, "rb");
   if (!f) {
@@ -95,7 +100,7 @@ bool load_file(const char *path, HostMlp *m, char *why, size_t whylen) {
   char magic[4];
   int hdr[5];
   bool ok = fread(magic, 1, 4, f) == 4 && memcmp(magic, "NMLS", 4) == 0 && fread(hdr, sizeof(int), 5, f) == 5 &&
-            hdr[0] == 1 && hdr[1] == kDin && hdr[3] == 1 && hdr[4] == kFam;
+            hdr[0] == kFileVersion && hdr[1] == kDin && hdr[3] == 1 && hdr[4] == kFam;
   if (!ok) {
     snprintf(why, whylen, "bad header");
     fclose(f);
@@ -105,11 +110,11 @@ bool load_file(const char *path, HostMlp *m, char *why, size_t whylen) {
     int n = 0;
     ok = fread(&n, sizeof(int), 1, f) == 1 && n == numc_gemm_config_count(fam);
     for (int i = 0; ok && i < n; i++) {
-      int d[6];
-      ok = fread(d, sizeof(int), 6, f) == 6;
+      int d[8];
+      ok = fread(d, sizeof(int), 8, f) == 8;
       const NumcKernelConfig &c = numc_gemm_config(fam, i)->cfg;
       ok = ok && d[0] == c.bm && d[1] == c.bn && d[2] == c.kc && d[3] == c.waves_m && d[4] == c.waves_n &&
-           d[5] == c.split_k;
+           d[5] == c.split_k && d[6] == c.waves_k && d[7] == c.direct;
     }
   }
   if (!ok) {
@@ -174,21 +179,32 @@ extern "C" NUMC_API int numc_selector_features(int family, int M, int N, int K, 
   if (!c || !out) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_features: no config %d in family %d", cfg, family);
   const double m = M > 0 ? M : 1, n = N > 0 ? N : 1, k = K > 0 ? K : 1;
   const double blocks = std::ceil(m / c->cfg.bm) * std::ceil(n / c->cfg.bn) * c->cfg.split_k;
-  out[0] = (float)(std::log2(m) / 16);
-  out[1] = (float)(std::log2(n) / 16);
-  out[2] = (float)(std::log2(k) / 16);
-  for (int f = 0; f < kFam; f++) out[3 + f] = f == family ? 1.f : 0.f;
-  out[8] = (flags & 1) ? 1.f : 0.f;
-  out[9] = (flags & 2) ? 1.f : 0.f;
-  out[10] = (flags & 4) ? 1.f : 0.f;
-  out[11] = (float)(std::log2((double)c->cfg.bm) / 8);
-  out[12] = (float)(std::log2((double)c->cfg.bn) / 8);
-  out[13] = (float)(std::log2((double)c->cfg.kc) / 8);
-  out[14] = (float)(std::log2((double)c->cfg.waves_m * c->cfg.waves_n) / 3);
-  out[15] = (float)(std::log2((double)c->cfg.split_k) / 3);
-  out[16] = (float)(std::log2(blocks) / 16);
-  out[17] = (float)(std::log2(std::ceil(blocks / 256.0)) / 8);
-  return 0;
+  int x = 0;
+  out[x++] = (float)(std::log2(m) / 16);
+  out[x++] = (float)(std::log2(n) / 16);
+  out[x++] = (float)(std::log2(k) / 16);
+  for (int f = 0; f < kFam; f++) out[x++] = f == family ? 1.f : 0.f;
+  out[x++] = (flags & 1) ? 1.f : 0.f;
+  out[x++] = (flags & 2) ? 1.f : 0.f;
+  out[x++] = (flags & 4) ? 1.f : 0.f;
+  out[x++] = (float)(std::log2((double)c->cfg.bm) / 8);
+  out[x++] = (float)(std::log2((double)c->cfg.bn) / 8);
+  out[x++] = (float)(std::log2((double)c->cfg.kc) / 8);
+  out[x++] = (float)(std::log2((double)c->cfg.waves_m * c->cfg.waves_n) / 3);
+  out[x++] = (float)(std::log2((double)c->cfg.split_k) / 3);
+  out[x++] = (float)(std::log2(blocks) / 16);
+  out[x++] = (float)(std::log2(std::ceil(blocks / 256.0)) / 8);
+  out[x++] = (float)(std::log2((double)c->cfg.waves_k) / 2);
+  out[x++] = c->cfg.direct ? 1.f : 0.f;
+  // whether the packed kernel reads an operand in place (16-byte aligned
+  // k-contiguous rows and K a multiple of the k-chunk; arrays start aligned)
+  const long long es = family == NUMC_FAMILY_F64 ? 8 : family == NUMC_FAMILY_I8 ? 1 : family >= NUMC_FAMILY_BF16 ? 2 : 4;
+  const long long pad = (flags & 4) ? 16 : 0;
+  const bool rows16 = (((long long)K + pad) * es) % 16 == 0;
+  const bool fam_ip = !c->cfg.direct && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8) && K % c->cfg.kc == 0;
+  out[x++] = fam_ip && c->inplace[0][0] && !(flags & 1) && rows16 ? 1.f : 0.f;
+  out[x++] = fam_ip && c->inplace[1][0] && (flags & 2) && rows16 ? 1.f : 0.f;
+  return x == kDin ? 0 : NUMC_SET_ERROR(NUMC_ERR_INTERNAL, "selector_features: %d features, expected %d", x, kDin);
 }
 
 extern "C" NUMC_API int numc_selector_feature_count(void) { return kDin; }
@@ -267,12 +283,61 @@ extern "C" NUMC_API int numc_selector_predict(int family, int M, int N, int K, i
   return 0;
 }
 
-int numc_selector_choose(int family, int M, int N, int K, int flags) {
+// Candidates for one call: the settled choice (an exact-shape entry, or the
+// model's best when it is confident); where the model's predictions of its
+// best few configurations lie within 2^0.1 of each other, those (at most 3)
+// in predicted order for the caller to time once; and every configuration
+// when the best predicted call takes under 2^-5 ms, where run-to-run noise is
+// of the order of the differences and timing them all costs little.  At most
+// `max`.  Returns the count, 0 without a trained model.  flags bits 0-2 are
+// the layout (what the model sees); higher bits only separate the
+// exact-shape entries.
+extern "C" NUMC_API int numc_selector_candidates(int family, int M, int N, int K, int flags, int allow_direct, int *out,
+                                                 int max) {
+  if (family < 0 || family >= NUMC_FAMILY_COUNT || !out || max < 1)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "selector_candidates: bad argument");
   std::lock_guard<std::mutex> lk(g_mu);
   ensure_loaded();
-  const ShapeKey key{family, flags, M, N, K};
-  auto tt = g_tuned.find(key);
-  if (tt != g_tuned.end()) return tt->second;
+  auto tt = g_tuned.find(ShapeKey{family, flags, M, N, K});
+  if (tt != g_tuned.end() && (allow_direct || !numc_gemm_config(family, tt->second)->cfg.direct)) {
+    out[0] = tt->second;
+    return 1;
+  }
+  if (!g_loaded) return 0;
+  const int n = numc_gemm_config_count(family);
+  float x[kDin];
+  std::vector<std::pair<float, int>> pred;
+  for (int i = 0; i < n; i++) {
+    if (!allow_direct && numc_gemm_config(family, i)->cfg.direct) continue;
+    numc_selector_features(family, M, N, K, flags & 7, i, x);
+    pred.push_back({g_model.forward(x), i});
+  }
+  if (pred.empty()) return 0;
+  std::sort(pred.begin(), pred.end());
+  int k = 1;
+  if (pred[0].first < -5.f) {
+    k = std::min((int)pred.size(), max);
+  } else {
+    while (k < (int)pred.size() && k < std::min(max, 3) && pred[k].first - pred[0].first <= 0.1f) k++;
+  }
+  for (int i = 0; i < k; i++) out[i] = pred[i].second;
+  return k;
+}
+
+// remember the measured winner of an exact shape (flags as in candidates)
+void numc_selector_record(int family, int M, int N, int K, int flags, int cfg) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_tuned();
+  g_tuned[ShapeKey{family, flags, M, N, K}] = cfg;
+}
+
+int numc_selector_choose(int family, int M, int N, int K, int flags, bool allow_direct) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_loaded();
+  // the cache keys carry whether direct configurations were allowed (bit 8)
+  const ShapeKey key{family, flags | (allow_direct ? 0 : 256), M, N, K};
+  auto tt = g_tuned.find(ShapeKey{family, flags, M, N, K});
+  if (tt != g_tuned.end() && (allow_direct || !numc_gemm_config(family, tt->second)->cfg.direct)) return tt->second;
   if (!g_loaded || family >= kFam) return -1;
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return it->second;
@@ -281,7 +346,8 @@ int numc_selector_choose(int family, int M, int N, int K, int flags) {
   int best = -1;
   float bt = 0;
   for (int i = 0; i < n; i++) {
-    numc_selector_features(family, M, N, K, flags, i, x);
+    if (!allow_direct && numc_gemm_config(family, i)->cfg.direct) continue;
+    numc_selector_features(family, M, N, K, flags & 7, i, x);
     float t = g_model.forward(x);
     if (best < 0 || t < bt) {
       best = i;
@@ -301,6 +367,21 @@ int numc_selector_choose(int family, int M, int N, int K, int flags) {
 // model, trained on single calls, favours small tiles that keep one problem's
 // GPU busy, which is not what a full batch needs.
 static int g_batch_thresh[NUMC_FAMILY_COUNT] = {64, 64, 64, 64, 64, 64, 64};
+
+// FP32 and int8 batches run the direct kernel until numc_matmul_batched_tune
+// measures otherwise; split-BF16 has only the packed one
+static bool g_batch_direct[NUMC_FAMILY_COUNT] = {true, false, false, false, true, false, false};
+
+bool numc_batch_direct(int family) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return family >= 0 && family < NUMC_FAMILY_COUNT && g_batch_direct[family];
+}
+
+void numc_batch_set_direct(int family, bool direct) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (family >= 0 && family < NUMC_FAMILY_COUNT)
+    g_batch_direct[family] = direct && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8);
+}
 
 int numc_batch_threshold(int family) {
   std::lock_guard<std::mutex> lk(g_mu);

@@ -207,6 +207,15 @@ INTS = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32]
 def test_int_configs_full_range(ctx, cfg, dt):
     rng = np.random.default_rng(500 + cfg)
     info = np.iinfo(dt)
+    if nc.config_get(nc.FAMILY_I8, cfg)["direct"] and dt not in (np.int8, np.uint8):
+        # the direct kernel reads one int8 digit plane; wider integers need the
+        # packed digit planes, and naming a direct configuration is refused
+        a = np.ones((4, 5), dt)
+        A, B = ctx.from_numpy(a), ctx.from_numpy(a.T.copy())
+        O = ctx.array((4, 4), nc.DTYPE_OF_NP[np.dtype(dt)], zeros=True)
+        assert nc.matmul(A, B, O, nc.METHOD_AUTO, cfg) == nc.ERR_VALUE
+        assert "one int8 digit plane" in nc.last_error()[1]
+        return
     for M, K, N in SHAPES:
         a = rng.integers(info.min, info.max, (M, K), dtype=dt, endpoint=True)
         b = rng.integers(info.min, info.max, (K, N), dtype=dt, endpoint=True)
@@ -554,6 +563,92 @@ def test_batched_graph_cache_alternating_batches(ctx):
     assert n == 1 and ms > 0
     for (a, b), O in zip(ref, Os):
         assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+
+
+def _set_schedule(fam, direct, thresh):
+    nc.check(nc.lib().numc_matmul_batched_set_schedule(fam, direct, thresh), "set schedule")
+
+
+@pytest.mark.parametrize("direct", [1, 0], ids=["direct", "packed"])
+@pytest.mark.parametrize("thresh", [0, 64, 1 << 30], ids=["all_large", "t64", "all_small"])
+def test_batched_f32_schedules_views(ctx, direct, thresh):
+    """both batched kernels and both tile classes on uneven problems whose
+    operands are row-major, stored transposed, or start one element past a
+    16-byte boundary; results against the oracle"""
+    rng = np.random.default_rng(19)
+    before = nc.batched_schedule(nc.FAMILY_F32)
+    _set_schedule(nc.FAMILY_F32, direct, thresh)
+    try:
+        As, Bs, Os, ref = [], [], [], []
+        for t, (M, N, K) in enumerate(zip(_logu(rng, 1, 400, 24), _logu(rng, 1, 400, 24), _logu(rng, 1, 600, 24))):
+            a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+            if t % 3 == 0:
+                A, B = ctx.from_numpy(a), ctx.from_numpy(b)
+            elif t % 3 == 1:   # stored transposed
+                A, B = ctx.from_numpy(a.T.copy()).transpose(), ctx.from_numpy(b.T.copy()).transpose()
+            else:              # one element past the aligned start, odd row strides
+                pa = np.zeros((M, K + 1), np.float32)
+                pa[:, 1:] = a
+                pb = np.zeros((K, N + 1), np.float32)
+                pb[:, 1:] = b
+                A, B = ctx.from_numpy(pa).slice(1, 1, K + 1), ctx.from_numpy(pb).slice(1, 1, N + 1)
+            As.append(A)
+            Bs.append(B)
+            Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+            ref.append((a, b))
+        for rep in range(2):   # capture, then replay
+            nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+            for (a, b), O in zip(ref, Os):
+                assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+    finally:
+        _set_schedule(nc.FAMILY_F32, *before)
+
+
+@pytest.mark.parametrize("odt,flags", [(np.int32, nc.OPT_WIDEN), (np.int8, 0)], ids=["widen", "wrap"])
+def test_batched_int8_direct_b_transposed(ctx, odt, flags):
+    """the direct batched kernel on int8 operands with k-contiguous rows (B
+    stored transposed), both tile classes; exact against the oracle"""
+    rng = np.random.default_rng(20)
+    before = nc.batched_schedule(nc.FAMILY_I8)
+    _set_schedule(nc.FAMILY_I8, 1, 64)
+    try:
+        As, Bs, Os, ref = [], [], [], []
+        for M, K, N in [(40, 300, 70), (128, 64, 128), (3, 5, 7), (260, 1000, 90), (65, 33, 200)]:
+            a = rng.integers(-128, 127, (M, K), dtype=np.int8, endpoint=True)
+            b = rng.integers(-128, 127, (K, N), dtype=np.int8, endpoint=True)
+            As.append(ctx.from_numpy(a))
+            Bs.append(ctx.from_numpy(b.T.copy()).transpose())
+            Os.append(ctx.array((M, N), nc.DTYPE_OF_NP[np.dtype(odt)], zeros=True))
+            ref.append(oracle.matmul_int(a, b, odt))
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os, flags=flags), "batched int8")
+        for want, O in zip(ref, Os):
+            assert np.array_equal(O.numpy(), want)
+    finally:
+        _set_schedule(nc.FAMILY_I8, *before)
+
+
+def test_batched_tune_picks_a_schedule(ctx):
+    """the tuner times the direct and packed kernels and keeps one; results
+    under the kept schedule stay exact"""
+    rng = np.random.default_rng(21)
+    before = nc.batched_schedule(nc.FAMILY_F32)
+    try:
+        As, Bs, Os, ref = [], [], [], []
+        for M, N, K in zip(_logu(rng, 16, 300, 40), _logu(rng, 16, 300, 40), _logu(rng, 16, 300, 40)):
+            a, b = rand_f32(rng, (M, K)), rand_f32(rng, (K, N))
+            As.append(ctx.from_numpy(a))
+            Bs.append(ctx.from_numpy(b))
+            Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+            ref.append((a, b))
+        t = nc.matmul_batched_tune(ctx, As, Bs, Os, nc.METHOD_PLAIN, 0, 2)
+        assert t >= 0
+        d, t2 = nc.batched_schedule(nc.FAMILY_F32)
+        assert t2 == t and d in (0, 1)
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+        for (a, b), O in zip(ref, Os):
+            assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+    finally:
+        _set_schedule(nc.FAMILY_F32, *before)
 
 
 def test_batched_int8_widen_and_wrap(ctx):

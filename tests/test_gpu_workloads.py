@@ -193,7 +193,9 @@ def test_selector_regret_heldout_retimed():
     L = _autotune_sigs()
     assert L.numc_selector_active() == 1, "the trained selector does not match the compiled configurations"
     buf = (C.c_float * 32)()
-    reg_mlp, reg_ana = [], []
+    cand = (C.c_int * 32)()
+    L.numc_selector_candidates.argtypes = [C.c_int] * 6 + [C.POINTER(C.c_int), C.c_int]
+    reg_mlp, reg_ana, reg_call = [], [], []
     with nc.Ctx() as ctx:
         for idx, fam, M, N, K, fl in _heldout_subset(160):
             ncfg = nc.config_count(fam)
@@ -206,14 +208,22 @@ def test_selector_regret_heldout_retimed():
             pick = L.numc_selector_choice(fam, M, N, K, fl)
             ana = L.numc_selector_heuristic(fam, M, N, K)
             assert 0 <= pick < ncfg and 0 <= ana < ncfg
+            # numc_matmul's rule: time the close candidates once, keep the fastest
+            nc_ = L.numc_selector_candidates(fam, M, N, K, fl, 1, cand, 32)
+            assert nc_ >= 1 and cand[0] == pick
+            call = min((cand[i] for i in range(nc_)), key=lambda c: best[c])
             reg_mlp.append(best[pick] / t_best - 1)
             reg_ana.append(best[ana] / t_best - 1)
+            reg_call.append(best[call] / t_best - 1)
         nc.check(nc.lib().numc_autotune_release(ctx.h), "numc_autotune_release")
-    m, a = float(np.mean(reg_mlp)), float(np.mean(reg_ana))
-    print("held-out regret (%d shapes): learned mean %.4f p95 %.4f, analytic mean %.4f p95 %.4f" %
-          (len(reg_mlp), m, np.percentile(reg_mlp, 95), a, np.percentile(reg_ana, 95)), flush=True)
+    m, a, c = float(np.mean(reg_mlp)), float(np.mean(reg_ana)), float(np.mean(reg_call))
+    print("held-out regret (%d shapes): learned mean %.4f p95 %.4f; with call timing mean %.4f p95 %.4f max %.3f; "
+          "analytic mean %.4f p95 %.4f" % (len(reg_mlp), m, np.percentile(reg_mlp, 95), c,
+                                          np.percentile(reg_call, 95), max(reg_call), a, np.percentile(reg_ana, 95)),
+          flush=True)
     assert m <= 0.05
     assert m < a
+    assert c <= m and np.percentile(reg_call, 95) <= 0.05
 
 
 # ---- the autotuner -------------------------------------------------------------------------
@@ -307,3 +317,71 @@ def test_selector_tune_pins_the_winner_and_keeps_results(ctx, fam):
         _check_family(fam, a, b, O.numpy())
         nc.check(L.numc_selector_pin(fam, M, N, K, flags, -1), "unpin")
     nc.check(nc.lib().numc_autotune_release(ctx.h), "numc_autotune_release")
+
+
+# ---- the direct (one-launch, in-place) configurations -------------------------------------
+
+def _misaligned(ctx, x, dt, transposed):
+    """a view holding x whose rows start one element past a 16-byte boundary
+    and whose row stride is not a multiple of 16 bytes (stored transposed
+    when asked: unit row stride, k stride of rows + 1 elements)"""
+    store = np.ascontiguousarray(x.T if transposed else x)
+    pad = np.zeros((store.shape[0], store.shape[1] + 1), store.dtype)
+    pad[:, 1:] = store
+    S = ctx.from_numpy(pad, dt if dt == nc.BFLOAT16 else None).slice(1, 1, store.shape[1] + 1)
+    if transposed:
+        S.transpose()
+    return S
+
+
+@pytest.mark.parametrize("fam", [nc.FAMILY_F32, nc.FAMILY_SPLIT_BF16, nc.FAMILY_F64, nc.FAMILY_I8, nc.FAMILY_BF16,
+                                 nc.FAMILY_F16], ids=["f32", "split_bf16", "f64", "i8", "bf16", "f16"])
+def test_direct_configs_misaligned_and_ragged(ctx, fam):
+    """every direct configuration on operand views at element (not 16-byte)
+    alignment, row-major and stored transposed, with K around the MFMA
+    fragment length (tails read nothing past K) and M, N off the tile"""
+    dt, method = FAM_CASE[fam]
+    rng = np.random.default_rng(90 + fam)
+    cfgs = [c for c in range(nc.config_count(fam)) if nc.config_get(fam, c)["direct"]]
+    assert cfgs
+    kf = nc.config_get(fam, cfgs[0])["kc"]
+    for cfg in cfgs:
+        for M, K, N in [(1, 1, 1), (33, kf - 1, 70), (70, kf + 1, 33), (65, 3 * kf + 5, 97), (5, 1000, 2)]:
+            if dt == nc.INT8:
+                a = rng.integers(-128, 127, (M, K), dtype=np.int8, endpoint=True)
+                b = rng.integers(-128, 127, (K, N), dtype=np.int8, endpoint=True)
+            elif dt == nc.BFLOAT16:
+                a = oracle.split_bf16(rng.standard_normal((M, K)).astype(np.float32))[0].reshape(M, K)
+                b = oracle.split_bf16(rng.standard_normal((K, N)).astype(np.float32))[0].reshape(K, N)
+            else:
+                npdt = {nc.FLOAT32: np.float32, nc.FLOAT64: np.float64, nc.FLOAT16: np.float16}[dt]
+                a, b = rng.standard_normal((M, K)).astype(npdt), rng.standard_normal((K, N)).astype(npdt)
+            for ta, tb in [(False, False), (True, True), (False, True)]:
+                A, B = _misaligned(ctx, a, dt, ta), _misaligned(ctx, b, dt, tb)
+                O = ctx.array((M, N), dt, zeros=True)
+                nc.check(nc.matmul(A, B, O, method, cfg), "direct matmul")
+                _check_family(fam, a, b, O.numpy())
+
+
+def test_direct_int8_widen_and_uint8(ctx):
+    """int8 -> int32 (the exact sum) and uint8 -> uint8 (mod 256) through the
+    direct configurations; int16 inputs fall back to packed digit planes under
+    the selector"""
+    rng = np.random.default_rng(95)
+    cfgs = [c for c in range(nc.config_count(nc.FAMILY_I8)) if nc.config_get(nc.FAMILY_I8, c)["direct"]]
+    for cfg in cfgs:
+        for M, K, N in [(40, 300, 70), (3, 5, 7), (129, 64, 31)]:
+            a = rng.integers(-128, 127, (M, K), dtype=np.int8, endpoint=True)
+            b = rng.integers(-128, 127, (K, N), dtype=np.int8, endpoint=True)
+            O = ctx.array((M, N), nc.INT32, zeros=True)
+            nc.check(nc.matmul(ctx.from_numpy(a), ctx.from_numpy(b), O, nc.METHOD_AUTO, cfg, nc.OPT_WIDEN), "widen")
+            assert np.array_equal(O.numpy(), oracle.matmul_int(a, b, np.int32))
+            ua, ub = a.view(np.uint8), b.view(np.uint8)
+            O = ctx.array((M, N), nc.UINT8, zeros=True)
+            nc.check(nc.matmul(ctx.from_numpy(ua), ctx.from_numpy(ub), O, nc.METHOD_AUTO, cfg), "uint8")
+            assert np.array_equal(O.numpy(), oracle.matmul_int(ua, ub))
+    a = rng.integers(-30000, 30000, (20, 40), dtype=np.int16)
+    b = rng.integers(-30000, 30000, (40, 10), dtype=np.int16)
+    O = ctx.array((20, 10), nc.INT16, zeros=True)
+    nc.check(nc.matmul(ctx.from_numpy(a), ctx.from_numpy(b), O), "int16 under the selector")
+    assert np.array_equal(O.numpy(), oracle.matmul_int(a, b))

@@ -96,7 +96,9 @@ def test_validation(ctx):
     assert nc.matmul(a, b, o) == nc.ERR_TYPE                        # bf16 @ f16
     b2 = ctx.array((3, 4), nc.BFLOAT16)
     assert nc.matmul(a, b2, o, nc.METHOD_SPLIT_BF16) == nc.ERR_TYPE  # methods need float32
-    assert nc.config_count(nc.FAMILY_BF16) == 8 and nc.config_count(nc.FAMILY_F16) == 8
+    # 8 packed configurations and 2 direct ones per 16-bit family
+    assert nc.config_count(nc.FAMILY_BF16) == 10 and nc.config_count(nc.FAMILY_F16) == 10
+    assert [nc.config_get(nc.FAMILY_BF16, i)["direct"] for i in range(10)] == [0] * 8 + [1] * 2
 
 
 # ---- GPU -----------------------------------------------------------------------

@@ -110,3 +110,9 @@ def test_selector_regret_on_held_out_shapes():
     assert r["mlp"]["mean_regret"] <= 0.05
     assert r["mlp"]["total_time_ratio"] <= 1.05
     assert r["mlp"]["mean_regret"] < r["analytic"]["mean_regret"]
+    # numc_matmul's rule: time the model's close candidates (all of them for
+    # calls under 2^-5 ms) once per shape -- p95 within 5 %, worst within 1.5x
+    fb = r["mlp_with_call_timing"]
+    assert fb["mean_regret"] <= r["mlp"]["mean_regret"]
+    assert fb["p95_regret"] <= 0.05 and fb["max_regret"] <= 0.5
+    assert 0 < fb["shapes_timed"] < r["shapes"]
