@@ -149,9 +149,11 @@ __device__ __attribute__((always_inline)) inline void direct_tile(const GemmArgs
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-  // this wave's k-fragments [f0, f1)
+  // this wave's k-fragments [f0, f1) (with WK = 1 each wave has the whole K,
+  // whichever wave of the block it is)
   const int nkf = (g.K + KF - 1) / KF;
-  const int f0 = (int)((long long)w * nkf / WK), f1 = (int)((long long)(w + 1) * nkf / WK);
+  const int wk = WK == 1 ? 0 : w;
+  const int f0 = (int)((long long)wk * nkf / WK), f1 = (int)((long long)(wk + 1) * nkf / WK);
   const int n = f1 - f0;
   const int ksub = (lane / T) * E;
   const unsigned char *pa[TM], *pb[TN];
