@@ -83,6 +83,22 @@ NUMC_API int numc_amd_gemm(NumcCtx *ctx, int dtype, int out_dtype, size_t M, siz
                            const void *a, size_t rsa, size_t csa, const void *b, size_t rsb, size_t csb,
                            void *c, size_t rso, size_t cso, const NumcMatmulOpts *opts);
 
+/* numc_amd_gemm for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in numc.  a, b and c are device addresses (as with
+ * NUMC_OPT_DEVICE_POINTERS), the kernels run on `stream` (a hipStream_t; NULL
+ * = the default stream) and return without synchronizing, and nothing is
+ * allocated: the workspace of the packed kernels comes from a 64 MiB device
+ * array the library holds on each GPU for the process's lifetime.  Problems
+ * whose every configuration needs more (packed operands of about 16M
+ * elements and up) fail with NUMC_ERR_VALUE; the selector's choice is used
+ * without call-time timing.  Calls that overlap on one GPU (other streams or
+ * threads) share that array and race.  64-bit integers are refused.  Result
+ * contract, opts and errors otherwise as numc_amd_gemm. */
+NUMC_API int numc_amd_gemm_noalloc(void *stream, int dtype, int out_dtype, size_t M, size_t N, size_t K,
+                                   const void *a, size_t rsa, size_t csa, const void *b, size_t rsb, size_t csb,
+                                   void *c, size_t rso, size_t cso, const NumcMatmulOpts *opts);
+
 /* Autotune the batched schedule for this batch: times the whole batch under
  * each candidate schedule and keeps the fastest for the batch's dtype family
  * in later numc_matmul_batched calls.  A schedule is the kernel -- direct

@@ -115,6 +115,9 @@ _SIGS = {
     "numc_amd_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t,
                                 _P, C.c_size_t, C.c_size_t, _P, C.c_size_t, C.c_size_t,
                                 _P, C.c_size_t, C.c_size_t, C.POINTER(MatmulOpts)]),
+    "numc_amd_gemm_noalloc": (C.c_int, [_P, C.c_int, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t,
+                                        _P, C.c_size_t, C.c_size_t, _P, C.c_size_t, C.c_size_t,
+                                        _P, C.c_size_t, C.c_size_t, C.POINTER(MatmulOpts)]),
     "numc_matmul_batched": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                       C.POINTER(MatmulOpts)]),
     "numc_matmul_batched_tune": (C.c_int, [_P, _SZ, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),

@@ -125,6 +125,12 @@ struct ScaleArgs {
 };
 int numc_scale_launch(NumcCtx *ctx, const ScaleArgs &a);
 int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p);
+// bytes of device scratch behind numc_amd_gemm_noalloc (per GPU, for the
+// process's lifetime)
+#define NUMC_NOALLOC_SCRATCH (64 << 20)
+// the kernel family of a dtype and method (matmul.cpp): a NumcFamily, 100 for
+// 64-bit integers, or a negative error code
+int numc_matmul_family_dt(int dtype, const NumcMatmulOpts *opts);
 // int8 digit planes of an integer problem (1 for the other families)
 int numc_problem_planes(const GemmProblem &p);
 // time the candidate configurations on the problem's own operands (autotune.hip)

@@ -70,6 +70,8 @@ static int pick_family_dt(NumcDType dt, const NumcMatmulOpts *o) {
 
 static int pick_family(const NumcArray *a, const NumcMatmulOpts *o) { return pick_family_dt(a->dtype, o); }
 
+int numc_matmul_family_dt(int dtype, const NumcMatmulOpts *opts) { return pick_family_dt((NumcDType)dtype, opts); }
+
 // ---- residency staging -----------------------------------------------------
 
 static int stage_in(NumcCtx *ctx, const NumcArray *x, const void **dptr) {
