@@ -2001,7 +2001,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   const bool pair = nz == 1 && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
                                 family == NUMC_FAMILY_FP8 || family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16 ||
                                 (family == NUMC_FAMILY_I8 && planes == 1 && p.in_dt == NUMC_DTYPE_INT8));
-  const bool masks = family == NUMC_FAMILY_I8 && !pair;
+  // (the plane masks matter only with several digit planes)
+  const bool masks = family == NUMC_FAMILY_I8 && !(planes == 1 && p.in_dt == NUMC_DTYPE_INT8);
   // A read in place by the GEMM (no pack of A): FP32 and single-plane int8,
   // A's rows k-contiguous and 16-byte aligned, K a multiple of the k-chunk
   // (every copy stays inside its row)
@@ -2031,14 +2032,17 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   pb.nRT = nRTb; pb.nKF = nKF; pb.dst = Bp; pb.plane_bytes = plb;
   pb.row_exp = expB; pb.flags = masks ? flags + NUMC_FLAG_INTS : nullptr;
   pb.src_zs = p.b_bs * (long long)ein; pb.dst_zs = plb * planes;
-  numc_prof_begin(ctx, PROF_PACK, family);
+  // (each pack launch is one profiled pack)
   if (adir && bdir) {
     // nothing to pack
   } else if (adir) {
+    numc_prof_begin(ctx, PROF_PACK, family);
     launch_pack(family, p.in_dt, planes, pb, (unsigned)nz, st);
   } else if (bdir) {
+    numc_prof_begin(ctx, PROF_PACK, family);
     launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
   } else if (pair) {
+    numc_prof_begin(ctx, PROF_PACK, family);
     const long long na = numc_batch_pack_blocks(family, pk), nb = numc_batch_pack_blocks(family, pb);
     const dim3 grid((unsigned)(na + nb));
     if (family == NUMC_FAMILY_F32) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F32>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
@@ -2048,7 +2052,10 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     else if (family == NUMC_FAMILY_F16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_I8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
   } else {
+    numc_prof_begin(ctx, PROF_PACK, family);
     launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
+    numc_prof_end(ctx);
+    numc_prof_begin(ctx, PROF_PACK, family);
     launch_pack(family, p.in_dt, planes, pb, (unsigned)nz, st);
   }
   numc_prof_end(ctx);

@@ -782,11 +782,33 @@ __global__ void k_copy16(const uint4 *src, uint4 *dst, long long n) {
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
 
+// [p, p + bytes) lies in one device allocation (not host memory, pinned or not)
+static int check_device_range(const void *p, size_t bytes, const char *what) {
+  hipPointerAttribute_t at{};
+  hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e != hipSuccess || at.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError(); // an unknown (host) pointer is not an error of later calls
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "device_copy: %s (%p) is not device memory", what, p);
+  }
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+    (void)hipGetLastError();
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "device_copy: %s (%p) has no device allocation", what, p);
+  }
+  if ((const char *)p + bytes > (const char *)base + size)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "device_copy: %zu bytes at %s run past its allocation", bytes, what);
+  return 0;
+}
+
 extern "C" NUMC_API int numc_device_copy(NumcCtx *ctx, void *dst, const void *src, size_t bytes) {
   if (!ctx || !dst || !src) return NUMC_SET_ERROR(NUMC_ERR_NULL, "device_copy: NULL argument");
   if (bytes % 16) return NUMC_SET_ERROR(NUMC_ERR_VALUE, "device_copy: size must be a multiple of 16");
+  if (((uintptr_t)dst | (uintptr_t)src) & 15)
+    return NUMC_SET_ERROR(NUMC_ERR_VALUE, "device_copy: dst and src must be 16-byte aligned");
   int err = numc_gpu_init(ctx);
   if (err) return err;
+  if ((err = check_device_range(src, bytes, "src")) || (err = check_device_range(dst, bytes, "dst"))) return err;
   long long n = (long long)(bytes / 16);
   unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 2048);
   if (n) hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)src, (uint4 *)dst, n);

@@ -204,3 +204,41 @@ def test_pointer_entry_error_contract(ctx):
     if not os.path.exists("/dev/kfd"):
         assert call(ctx.h, nc.FLOAT32, nc.FLOAT32, 4, 5, 3, pa, pb, pc) == nc.ERR_INTERNAL
         assert np.array_equal(c, np.zeros((4, 5), np.float32)), "no CPU fallback"
+
+
+def test_dtype_header_keeps_numc_generators(tmp_path):
+    """include/numc/dtype.h keeps numc's element-type macros, X-macro
+    generators and numc_dtype_align: a C file instantiating a table through
+    GENERATE_NUMC_TYPES compiles and agrees with numc_dtype_size; where numc's
+    sources are present, numc's own src/matmul/kernel.h (which expands
+    GENERATE_NUMC_TYPES) compiles against this header too."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    src = tmp_path / "gen.c"
+    src.write_text(
+        "#include <numc/dtype.h>\n#include <stdio.h>\n"
+        "#define ROW(E, T) {E, sizeof(T), NUMC_ALIGNOF(T)},\n"
+        "static const struct { int e; size_t s, a; } t[] = {GENERATE_NUMC_TYPES(ROW)};\n"
+        "#define ONE(E, T) + 1\n"
+        "int main(void) {\n"
+        "  int bad = 0, n = sizeof t / sizeof t[0];\n"
+        "  for (int i = 0; i < n; i++)\n"
+        "    bad += t[i].e != i || t[i].s != numc_dtype_size((NumcDType)i) || t[i].a != numc_dtype_align((NumcDType)i);\n"
+        "  bad += (0 GENERATE_INT_NUMC_TYPES(ONE)) != 8 || (0 GENERATE_FLOAT_NUMC_TYPES(ONE)) != 2;\n"
+        "  bad += (0 GENERATE_32BIT_NUMC_TYPES(ONE)) != 3 || (0 GENERATE_64BIT_NUMC_TYPES(ONE)) != 3;\n"
+        "  printf(\"%d %d\\n\", n, bad);\n  return bad;\n}\n")
+    exe = tmp_path / "gen"
+    subprocess.run([cc, "-std=gnu11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.split() == ["10", "0"], out.stdout
+    ref = "/root/reference"
+    clang = "/opt/rocm/lib/llvm/bin/clang"
+    if os.path.exists(os.path.join(ref, "src", "matmul", "kernel.h")) and os.path.exists(clang):
+        k = tmp_path / "kernel_user.c"
+        k.write_text('#include "matmul/kernel.h"\nint main(void) { return 0; }\n')
+        r = subprocess.run([clang, "-std=gnu23", "-fsyntax-only", "-fopenmp", "-I", os.path.join(ROOT, "include"),
+                            "-I", os.path.join(ref, "src"), str(k)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]

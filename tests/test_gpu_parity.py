@@ -859,3 +859,13 @@ def test_device_copy(ctx):
         assert np.array_equal(Y.numpy(), x), nbytes
     assert nc.lib().numc_device_copy(ctx.h, dst, src, 24) == nc.ERR_VALUE
     assert nc.lib().numc_device_copy(ctx.h, None, src, 16) == nc.ERR_NULL
+    # refused before any launch: host memory, misalignment, a range past the allocation
+    H = ctx.array((64,), nc.UINT8, zeros=True)        # host-current: its data pointer is (pinned) host memory
+    assert nc.lib().numc_device_copy(ctx.h, nc.lib().numc_array_data(H.h), src, 16) == nc.ERR_VALUE
+    assert "not device memory" in nc.last_error()[1]
+    hostbuf = np.zeros(64, np.uint8)
+    assert nc.lib().numc_device_copy(ctx.h, dst, hostbuf.ctypes.data, 16) == nc.ERR_VALUE
+    assert nc.lib().numc_device_copy(ctx.h, dst + 4, src, 16) == nc.ERR_VALUE
+    assert nc.lib().numc_device_copy(ctx.h, dst, src, 1 << 40) == nc.ERR_VALUE
+    assert "past its allocation" in nc.last_error()[1]
+    nc.check(nc.lib().numc_device_copy(ctx.h, dst, src, 16), "numc_device_copy after refusals")

@@ -300,5 +300,13 @@ def test_bmm_operands_in_place(ctx, dt, b_transposed):
     exact = np.matmul(a.astype(np.int64), bt.astype(np.int64))
     want = exact.astype(np.int8) if dt == np.int8 else exact.astype(np.float32)
     O = ctx.array(want.shape, DT[dt], zeros=True)
-    nc.check(nc.bmm(ctx.from_numpy(a), B, O), "bmm in place")
+    A = ctx.from_numpy(a)
+    fam = nc.FAMILY_I8 if dt == np.int8 else nc.FAMILY_F32
+    ctx.profile(True)
+    ctx.profile_read(0, fam)
+    nc.check(nc.bmm(A, B, O), "bmm in place")
+    packs = ctx.profile_read(0, fam)[1]
+    ctx.profile(False)
     assert np.array_equal(O.numpy(), want)
+    # which path ran: a is never packed; b is packed unless float32 reads it in place too
+    assert packs == (0 if (b_transposed and dt == np.float32) else 1), packs
