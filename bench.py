@@ -214,7 +214,12 @@ def time_whole_and_kernels(nc, ctx, fn, family, steps, warmup, batched=False):
         k[name] = (ms / n if n else 0.0, n / steps)
     ctx.profile(False)
     main = k["batched_gemm"] if batched else k["gemm"]
-    return {"ms_per_call": whole, "kernel_ms": main[0], "kernel_launches_per_call": main[1],
+    kernel_ms = main[0]
+    if not batched and main[1] == 1 and k["pack"][1] == 0 and k["reduce"][1] == 0:
+        # a one-launch call: its kernel takes no longer than the whole call timed
+        # without per-kernel events (the events add their own gap to a short kernel)
+        kernel_ms = min(kernel_ms, whole)
+    return {"ms_per_call": whole, "kernel_ms": kernel_ms, "kernel_launches_per_call": main[1],
             "pack_ms": k["pack"][0], "pack_launches_per_call": k["pack"][1],
             "reduce_ms": k["reduce"][0], "reduce_launches_per_call": k["reduce"][1]}
 
@@ -278,7 +283,7 @@ def bench_configs(nc, steps, warmup, cores, with_cpu):
     return res
 
 
-def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=True):
+def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=True, schedule=None, separate=True):
     """4096 problems, M, N, K log-uniform integers in [16, 1024], U[-1,1) values
     (int8: integers in [-128, 127]): one persistent graph-replayed launch vs one
     numc_matmul_ex call per problem, the batched GEMM kernel's roofline, and
@@ -322,10 +327,14 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
             Os.append(O)
         # autotuned schedule: the kernel (direct or packed) and the large-tile
         # threshold, measured on this batch
-        thresh = nc.matmul_batched_tune(ctx, As, Bs, Os, method, flags, 3)
-        if thresh < 0:
-            nc.check(thresh, "numc_matmul_batched_tune")
-        direct, _ = nc.batched_schedule(fam)
+        if schedule is None:
+            thresh = nc.matmul_batched_tune(ctx, As, Bs, Os, method, flags, 3)
+            if thresh < 0:
+                nc.check(thresh, "numc_matmul_batched_tune")
+        else:   # a fixed schedule (profiling runs): the direct kernel where the family has one
+            want_direct = schedule[0] and fam in (nc.FAMILY_F32, nc.FAMILY_I8)
+            nc.check(nc.lib().numc_matmul_batched_set_schedule(fam, int(want_direct), schedule[1]), "schedule")
+            thresh = schedule[1]
         H = nc.BatchHandles(As, Bs, Os)   # the handle arrays, built once as a C caller keeps them
         nc.check(nc.matmul_batched(ctx, H, method=method, flags=flags), "batched warmup")  # builds the graph
         ctx.synchronize()
@@ -337,15 +346,18 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
         t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul_batched(ctx, H, method=method, flags=flags), fam,
                                    max(2, steps // 2), 1, batched=True)
         # one call per problem, same stream, no graph
-        for i in range(count):
-            nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate warmup")
-        ctx.synchronize()
-        ctx.mark(4)
-        for i in range(count):
-            nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate")
-        ctx.mark(5)
-        t_sep = ctx.elapsed_ms(4, 5)
+        t_sep = float("nan")
+        if separate:
+            for i in range(count):
+                nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate warmup")
+            ctx.synchronize()
+            ctx.mark(4)
+            for i in range(count):
+                nc.check(nc.matmul(As[i], Bs[i], Os[i], method, -1, flags), "separate")
+            ctx.mark(5)
+            t_sep = ctx.elapsed_ms(4, 5)
         nbytes = float(np.sum(ein * (M_ * K_ + K_ * N_) + eout * M_ * N_))
+        direct = t["pack_launches_per_call"] == 0   # what ran: the direct kernel packs nothing
         e = {"ms_batched": round(t_batched, 4), "tflops_batched": round(flop / t_batched * 1e-9, 2),
              "tile_threshold": thresh, "kernel": "direct" if direct else "packed",
              "ms_separate_launches": round(t_sep, 3),
@@ -564,6 +576,11 @@ def main():
     ap.add_argument("--suite", action="store_true",
                     help="time the other configurations (f32 512^3, f64 2048^3, int32 2048^3, int8 4096^3, "
                          "bf16/f16 8192^3, numc_linear, numc_bmm)")
+    ap.add_argument("--no-separate", action="store_true",
+                    help="batched runs: skip the one-call-per-problem comparison (profiling runs)")
+    ap.add_argument("--batch-schedule", default=None,
+                    help="batched runs: a fixed schedule instead of the tuner's, 'direct:T' or 'packed:T' "
+                         "(T = the large-tile threshold; split-BF16 always runs packed)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the reference copy kernel over this many MiB (HBM counter calibration)")
     ap.add_argument("--vendor", action="store_true",
@@ -596,10 +613,15 @@ def main():
         bench_suite(nc, args.steps, args.warmup, args.seed)
         return
     cores = min(16, os.cpu_count() or 1)
+    sched = None
+    if args.batch_schedule:
+        kind, t = args.batch_schedule.split(":")
+        sched = (kind == "direct", int(t))
     if args.only == "batched":
         log("bench: batched uneven GEMM, %d problems" % args.batch)
         print(json.dumps(bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2), args.steps,
-                                       args.warmup, cores, not args.no_cpu)), flush=True)
+                                       args.warmup, cores, not args.no_cpu, sched, not args.no_separate)),
+              flush=True)
         return
     if args.only == "configs":
         print(json.dumps({"configs": bench_configs(nc, args.steps, args.warmup, cores, not args.no_cpu),
