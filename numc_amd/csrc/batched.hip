@@ -243,11 +243,13 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       d.rso = p.rso;
       d.cso = p.cso;
       d.out_bytes = (int)eout;
-      const int cls = (p.M > thresh && p.N > thresh) ? 0 : 1, t = cls == 0 ? 64 : 32;
-      // a tile's time: its multiply-adds plus a fixed start-up (one wave per
-      // SIMD streams K / 4 of them), so long-K tiles go first
-      const double tile_time = (double)t * t * (p.K > 0 ? p.K : 1) + 2.0e5;
-      const int nbm = (int)cdivll(p.M, t), nbn = (int)cdivll(p.N, t);
+      // each dimension above the threshold takes 64-wide tiles, else 32
+      const int tm = p.M > thresh ? 64 : 32, tn = p.N > thresh ? 64 : 32;
+      const int cls = (tm == 64 ? 2 : 0) | (tn == 64 ? 1 : 0);
+      // a tile's time: its multiply-adds plus a fixed start-up, so long-K
+      // tiles go first
+      const double tile_time = (double)tm * tn * (p.K > 0 ? p.K : 1) + 2.0e5;
+      const int nbm = (int)cdivll(p.M, tm), nbn = (int)cdivll(p.N, tn);
       for (int x = 0; x < nbm; x++)
         for (int y = 0; y < nbn; y++) work.push_back({tile_time, make_int4((int)i, x, y, cls)});
     }

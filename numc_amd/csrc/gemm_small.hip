@@ -311,8 +311,9 @@ __global__ void __launch_bounds__(WK * 64) k_gemm_direct(const GemmArgs g) {
 // wave is on its own: it takes an item from an atomic counter and computes
 // that tile over the whole K (no LDS, no barriers), so a wave's load pipeline
 // runs the full length of its K and the waves of a block never wait for each
-// other.  An item names its problem and tile class (0: 64 x 64, 1: 32 x 32);
-// the queue is ordered longest-first.  Each problem's k strides pick whole
+// other.  An item names its problem and tile class (64 or 32 rows, 64 or 32
+// columns: each dimension above the schedule's threshold takes 64); the queue
+// is ordered longest-first.  Each problem's k strides pick whole
 // 16-byte loads or element loads.
 // ---------------------------------------------------------------------------
 template <class MF, int EPI, int TM, int TN, int PF>
@@ -359,8 +360,13 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_batched_direct(const BatchDir
     g.out_bytes = pr.out_bytes;
     g.nprod = 1;
     const bool ak = pr.a_ks == es, bk = pr.b_ks == es;
-    if (w.w == 0) batch_wave_item<MF, EPI, 2, 2, 3>(g, w.y, w.z, ak, bk);
-    else batch_wave_item<MF, EPI, 1, 1, 4>(g, w.y, w.z, ak, bk);
+    // class: bit 1 = 64 rows (else 32), bit 0 = 64 columns (else 32)
+    switch (w.w) {
+    case 3: batch_wave_item<MF, EPI, 2, 2, 3>(g, w.y, w.z, ak, bk); break;
+    case 2: batch_wave_item<MF, EPI, 2, 1, 3>(g, w.y, w.z, ak, bk); break;
+    case 1: batch_wave_item<MF, EPI, 1, 2, 3>(g, w.y, w.z, ak, bk); break;
+    default: batch_wave_item<MF, EPI, 1, 1, 4>(g, w.y, w.z, ak, bk); break;
+    }
   }
 }
 
