@@ -429,10 +429,10 @@ __device__ inline void flags_or(int *flags, int mask) {
   if (mask & ~cur) atomicOr(slot, mask);
 }
 
+// one lane slot t of every plane, any strides (the whole wave calls it: the
+// plane mask takes a ballot)
 template <class S, int P>
-__global__ void k_pack_digits(PackArgs a) {
-  zslice(a);
-  long long t = blockIdx.x * 256LL + threadIdx.x;
+__device__ __attribute__((always_inline)) inline void pack_digits_unit(const PackArgs &a, long long t) {
   bool live = t < (long long)a.nRT * a.nKF * 64;
   uint32_t w[P][4];
 #pragma unroll
@@ -464,6 +464,12 @@ __global__ void k_pack_digits(PackArgs a) {
   for (int p = 0; p < P; p++)
     if (__ballot((w[p][0] | w[p][1] | w[p][2] | w[p][3]) != 0)) mask |= 1 << p;
   if ((threadIdx.x & 63) == 0 && mask && a.flags) flags_or(a.flags, mask); // no masks: null
+}
+
+template <class S, int P>
+__global__ void k_pack_digits(PackArgs a) {
+  zslice(a);
+  pack_digits_unit<S, P>(a, blockIdx.x * 256LL + threadIdx.x);
 }
 
 // ---- FP8 scales: row maxima of |x| as float bits, then exponents ----------
@@ -1425,6 +1431,46 @@ __global__ void __launch_bounds__(256) k_pack_digits_tiled_rows(PackArgs a) {
   digits_tile_rows<S, P>(a, blockIdx.x, tile);
 }
 
+// blocks of one operand's digit pack and the walk it takes (mode 0: k-contiguous
+// tiles, 1: row-contiguous tiles, 2: one thread per lane slot)
+static long long digits_blocks(PackArgs &pk) {
+  if (pk.ks == 1) {
+    pk.mode = 0;
+    return (long long)pk.nRT * ((pk.nKF * 32LL + 127) / 128);
+  }
+  if (pk.rs == 1) {
+    pk.mode = 1;
+    return ((pk.nRT + 3) / 4) * (long long)pk.nKF;
+  }
+  pk.mode = 2;
+  return ((long long)pk.nRT * pk.nKF * 64 + 255) / 256;
+}
+
+// both operands' digit planes in one launch: blocks [0, na) pack A, the rest B
+template <class S, int P>
+__global__ void __launch_bounds__(256) k_pack_pair_digits(const PackArgs a, const PackArgs b, unsigned na) {
+  __shared__ uint32_t tile[PACK_TILE_WORDS];
+  const bool isa = blockIdx.x < na;
+  const PackArgs &x = isa ? a : b;
+  const long long u = isa ? blockIdx.x : blockIdx.x - na;
+  if (x.mode == 0) digits_tile_k<S, P>(x, u, tile);
+  else if (x.mode == 1) digits_tile_rows<S, P>(x, u, tile);
+  else pack_digits_unit<S, P>(x, u * 256 + threadIdx.x);
+}
+
+template <class S, int P> static void launch_pair_digits_p(PackArgs &pa, PackArgs &pb, hipStream_t st) {
+  const long long na = digits_blocks(pa), nb = digits_blocks(pb);
+  hipLaunchKernelGGL((k_pack_pair_digits<S, P>), dim3((unsigned)(na + nb)), dim3(256), 0, st, pa, pb, (unsigned)na);
+}
+
+template <class S> static void launch_pair_digits(int planes, PackArgs &pa, PackArgs &pb, hipStream_t st) {
+  switch (planes) {
+  case 1: launch_pair_digits_p<S, 1>(pa, pb, st); break;
+  case 2: launch_pair_digits_p<S, 2>(pa, pb, st); break;
+  default: launch_pair_digits_p<S, 4>(pa, pb, st); break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // one-byte operands with one digit plane (int8 and uint8 results): a digit is
 // the byte itself, so a fragment lane is 16 source bytes, copied as is
@@ -2051,6 +2097,17 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
     else if (family == NUMC_FAMILY_BF16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_BF16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else if (family == NUMC_FAMILY_F16) hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_F16>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
     else hipLaunchKernelGGL((k_pack_pair<NUMC_FAMILY_I8>), grid, dim3(256), 0, st, pk, pb, (unsigned)na);
+  } else if (family == NUMC_FAMILY_I8 && nz == 1) {
+    // wider integers (several digit planes): both operands in one launch too
+    numc_prof_begin(ctx, PROF_PACK, family);
+    switch (p.in_dt) {
+    case NUMC_DTYPE_UINT8: launch_pair_digits<uint8_t>(planes, pk, pb, st); break;
+    case NUMC_DTYPE_INT16: launch_pair_digits<int16_t>(planes, pk, pb, st); break;
+    case NUMC_DTYPE_UINT16: launch_pair_digits<uint16_t>(planes, pk, pb, st); break;
+    case NUMC_DTYPE_INT32: launch_pair_digits<int32_t>(planes, pk, pb, st); break;
+    case NUMC_DTYPE_UINT32: launch_pair_digits<uint32_t>(planes, pk, pb, st); break;
+    default: launch_pair_digits<int8_t>(planes, pk, pb, st); break;
+    }
   } else {
     numc_prof_begin(ctx, PROF_PACK, family);
     launch_pack(family, p.in_dt, planes, pk, (unsigned)nz, st);
