@@ -23,6 +23,15 @@ roofline: the MFMA kernel of the split-BF16 method; its average duration comes
 from HIP events the library records on its own stream around each launch.
 cpu_baseline: numc's own numc_matmul (oracle/_ref, built from numc's sources,
 AVX2 + OpenMP) on the same bytes, min of 2 after a warmup, on this host.
+
+configs: the other workloads, each with its own roofline and cpu_baseline --
+float32 512^3, float64 2048^3, int32 2048^3 and int8 4096^3 through
+numc_matmul (whole call, plus the library's per-launch kernel times), and the
+uneven batch of 4096 problems (M, N, K log-uniform in [16, 1024]) through
+numc_matmul_batched in float32, split-BF16 and int8 -> int32, after
+numc_matmul_batched_tune picks its schedule.  host_cpu names the host's CPU
+model, cores and ISA flags.  --no-configs leaves them out; --only configs or
+--only batched runs just those.
 """
 import argparse
 import json

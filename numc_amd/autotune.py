@@ -11,7 +11,7 @@ uniform over the seven families, and layout flags (20% A stored transposed,
 20% B stored transposed, 10% padded rows).  Each shape's every configuration
 is timed by numc_autotune_shape (whole calls on device scratch operands).
 
-train: the MLP cost model (18 features -> 64 -> 64 -> log2 ms) is trained by
+train: the MLP cost model (24 features -> 64 -> 64 -> log2 ms) is trained by
 the library's HIP kernels (numc_mlp_train) on the training shapes.
 
 eval: regret = t(chosen) / t(best measured) - 1 over the held-out shapes, for

@@ -588,6 +588,9 @@ extern "C" NUMC_API NumcArray *numc_array_copy(const NumcArray *arr) {
   return c;
 }
 
+// reshape: numc src/array_core.c:456-497 (same checks, codes and messages).
+// One check more: a non-contiguous view is refused with NUMC_ERR_SHAPE, where
+// numc would give it dense strides over the view's data (other elements).
 extern "C" NUMC_API int numc_array_reshape(NumcArray *arr, const size_t *new_shape, size_t new_dim) {
   if (!arr || !new_shape || new_dim == 0) {
     NUMC_SET_ERROR(NUMC_ERR_NULL, "reshape: NULL or invalid args (arr=%p new_shape=%p new_dim=%zu)",
@@ -617,6 +620,7 @@ extern "C" NUMC_API int numc_array_reshape(NumcArray *arr, const size_t *new_sha
   return 0;
 }
 
+// numc src/array_core.c:499-521: a dense copy, then reshape
 extern "C" NUMC_API NumcArray *numc_array_reshape_copy(const NumcArray *arr, const size_t *new_shape,
                                                        size_t new_dim) {
   if (!arr || !new_shape || new_dim == 0) {
@@ -628,6 +632,7 @@ extern "C" NUMC_API NumcArray *numc_array_reshape_copy(const NumcArray *arr, con
   return c;
 }
 
+// transpose: numc src/array_core.c:523-560 (permutes shape and strides; no data moves)
 extern "C" NUMC_API int numc_array_transpose(NumcArray *arr, const size_t *axes) {
   if (!arr || !axes) {
     NUMC_SET_ERROR(NUMC_ERR_NULL, "transpose: NULL arg (arr=%p axes=%p)", (void *)arr, (const void *)axes);
@@ -655,6 +660,7 @@ extern "C" NUMC_API int numc_array_transpose(NumcArray *arr, const size_t *axes)
   return 0;
 }
 
+// numc src/array_core.c:562-581: a dense copy, then transpose
 extern "C" NUMC_API NumcArray *numc_array_transpose_copy(const NumcArray *arr, const size_t *axes) {
   if (!arr || !axes) {
     NUMC_SET_ERROR(NUMC_ERR_NULL, "transpose_copy: NULL arg");
@@ -665,6 +671,8 @@ extern "C" NUMC_API NumcArray *numc_array_transpose_copy(const NumcArray *arr, c
   return c;
 }
 
+// slice: numc src/array_core.c:583-650 (a view sharing the parent's data;
+// start/stop/step checks and codes as numc's)
 extern "C" NUMC_API NumcArray *numc_array_slice(const NumcArray *arr, NumcSlice *sl) {
   if (!arr || !sl) {
     NUMC_SET_ERROR(NUMC_ERR_NULL, "slice: NULL arg (arr=%p slice=%p)", (const void *)arr, (void *)sl);
