@@ -19,7 +19,8 @@ typedef enum {
   NUMC_METHOD_AUTO = 0,       /* = PLAIN for float32; the only choice for other dtypes */
   NUMC_METHOD_PLAIN = 1,      /* FP32 MFMA, FP32 accumulate */
   NUMC_METHOD_SPLIT_BF16 = 2, /* a = hi + lo (bf16 each), hi*hi + hi*lo + lo*hi, FP32 accumulate */
-  NUMC_METHOD_FP8_SCALED = 3, /* per-row (A) / per-column (B) power-of-two scales, OCP e4m3, FP32 accumulate */
+  NUMC_METHOD_FP8_SCALED = 3, /* OCP e4m3 with a power-of-two scale per 32 consecutive k of each row of A and
+                                 column of B (the MFMA's E8M0 block scales), FP32 accumulate */
 } NumcMethod;
 
 typedef struct {

@@ -48,7 +48,7 @@ struct PackArgs {
   int nRT, nKF;     // padded row-fragment and k-fragment counts
   unsigned char *dst;
   long long plane_bytes;
-  const int *row_exp; // FP8
+  unsigned char *sc;  // FP8: block scale bytes, [row tile][k-fragment pair][64]
   int *flags;         // integer digit planes: nonzero mask
   int mode;           // batched pack: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 lane slots,
                       // 3 int8 byte tiles (k-contiguous), 4 int8 byte tiles (row-contiguous)

@@ -20,27 +20,8 @@ static inline unsigned blocks_for(long long n) { return (unsigned)cdiv(n, 256); 
 // number formats (device)
 // ===========================================================================
 
-// binary32 -> OCP e4m3fn, round to nearest even, saturating to +-448
-__device__ inline uint32_t e4m3_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  uint32_t sign = (u >> 24) & 0x80u;
-  uint32_t a = u & 0x7FFFFFFFu;
-  if (a > 0x7F800000u) return sign | 0x7Fu;
-  if (a >= 0x43E00000u) return sign | 0x7Eu;
-  int e = (int)(a >> 23) - 127;
-  uint32_t code;
-  if (e < -6) {
-    code = (uint32_t)__builtin_rintf(__uint_as_float(a) * 512.0f);
-  } else {
-    uint32_t m = a & 0x7FFFFFu, keep = m >> 20, rem = m & 0xFFFFFu;
-    if (rem > 0x80000u || (rem == 0x80000u && (keep & 1u))) keep++;
-    if (keep == 8u) { keep = 0; e++; }
-    code = ((uint32_t)(e + 7) << 3) | keep;
-    if (code > 0x7Eu) code = 0x7Eu;
-  }
-  return sign | code;
-}
-
+// the FP8 block exponent: maxabs * 2^ex lies in [128, 256) (0 for an
+// all-zero block), clamped to +-126 so that the E8M0 byte 127 - ex is finite
 __device__ inline int fp8_scale_exp(float maxabs) {
   if (!(maxabs > 0.0f)) return 0;
   int q;
@@ -180,22 +161,53 @@ __global__ void k_pack_split_bf16(PackArgs a) {
 }
 
 
-// f32 * 2^row_exp[row] -> e4m3  (T=32, 16 per lane, KF=32)
+// FP8 block scales.  Each row of each packed fragment -- 32 consecutive k of
+// one row, 32-aligned -- gets its own power-of-two scale: the values are
+// multiplied by 2^ex (ex = fp8_scale_exp of the block's max |x|, so that the
+// block's largest magnitude lands in [128, 256)) and rounded to e4m3, and the
+// GEMM's MFMA multiplies the block back by the E8M0 scale 127 - ex.  The two
+// lane halves of a fragment row (lanes r and r + 32, 16 values each) combine
+// their maxima with one cross-half exchange, so every caller runs all 64
+// lanes of a wave together.  Scale bytes: [row tile][k-fragment pair][64],
+// byte r + 32*(kf & 1) for row r of k-fragment kf -- the order in which the
+// MFMA takes them (lane r scales fragment 2p, lane r + 32 fragment 2p + 1).
+__device__ __attribute__((always_inline)) inline void fp8_block_unit(const PackArgs &a, long long rt, int kf, int lane,
+                                                                     const float *x, unsigned char *dst16) {
+  float m = 0.f;
+#pragma unroll
+  for (int e = 0; e < 16; e++) m = fmaxf(m, fabsf(x[e]));
+  m = fmaxf(m, __shfl_xor(m, 32));
+  const int ex = fp8_scale_exp(m);
+  // v_cvt_pk_fp8_f32 rounds to nearest even into OCP e4m3.  Below 448 in
+  // magnitude (a scaled block lies below 256) it matched a software RNE
+  // encoder on every bf16-representable input, every e4m3 rounding tie with
+  // its float neighbours and 4M random values.  (A non-finite input makes its
+  // block's codes NaN.)
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(x[4 * q], ex), ldexpf(x[4 * q + 1], ex), 0, false);
+    w[q] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(x[4 * q + 2], ex), ldexpf(x[4 * q + 3], ex), lo, true);
+  }
+  *(uint4 *)dst16 = make_uint4(w[0], w[1], w[2], w[3]);
+  if (lane < 32) a.sc[(rt * (a.nKF >> 1) + (kf >> 1)) * 64 + lane + 32 * (kf & 1)] = (unsigned char)(127 - ex);
+}
+
+// one 16-byte lane slot of an FP8 fragment, any strides (T=32, 16 per lane, KF=32)
 __device__ inline void pack_unit_fp8(const PackArgs &a, long long t) {
   int lane = (int)(t & 63);
   long long fr = t >> 6;
   int kf = (int)(fr % a.nKF);
-  long long row = (fr / a.nKF) * 32 + (lane & 31);
+  long long rt = fr / a.nKF;
+  long long row = rt * 32 + (lane & 31);
   int k0 = kf * 32 + (lane >> 5) * 16;
-  int ex = row < a.R ? a.row_exp[row] : 0;
-  uint32_t w[4] = {0, 0, 0, 0};
+  float x[16];
 #pragma unroll
   for (int e = 0; e < 16; e++) {
     int k = k0 + e;
-    float x = (row < a.R && k < a.K) ? ld<float>(a.src, row * a.rs + (long long)k * a.ks) : 0.f;
-    w[e >> 2] |= e4m3_rne(ldexpf(x, ex)) << (8 * (e & 3));
+    x[e] = (row < a.R && k < a.K) ? ld<float>(a.src, row * a.rs + (long long)k * a.ks) : 0.f;
   }
-  *(uint4 *)(a.dst + t * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+  fp8_block_unit(a, rt, kf, lane, x, a.dst + t * 16);
 }
 
 __global__ void k_pack_fp8(PackArgs a) {
@@ -207,7 +219,7 @@ __global__ void k_pack_fp8(PackArgs a) {
 // ---- tiled packs for f32 rows with unit k-stride ------------------------
 // A block stages a 32-row x 128-k tile through LDS: reads are whole 512-byte
 // row segments (coalesced), writes are whole 1 KiB fragments.  MODE_PK: 0
-// plain f32 (KF 8), 1 bf16 hi/lo (KF 16), 2 e4m3 with row exponents (KF 32).
+// plain f32 (KF 8), 1 bf16 hi/lo (KF 16), 2 e4m3 with block scales (KF 32).
 // LDS words one pack tile needs (every tiled pack below fits)
 #define PACK_TILE_WORDS (128 * 36)
 
@@ -281,12 +293,8 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
       *(uint4 *)(a.dst + off) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
       *(uint4 *)(a.dst + a.plane_bytes + off) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
     } else {
-      const long long row = row0 + r;
-      const int ex = row < a.R ? a.row_exp[row] : 0;
-      uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int e = 0; e < 16; e++) w[e >> 2] |= e4m3_rne(ldexpf(t[e], ex)) << (8 * (e & 3));
-      *(uint4 *)(a.dst + off) = make_uint4(w[0], w[1], w[2], w[3]);
+      // (a wave's 64 units share f, so the lane halves meet in fp8_block_unit)
+      fp8_block_unit(a, rt, kf, lane, t, a.dst + off);
     }
   }
 }
@@ -379,12 +387,7 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
       *(uint4 *)(a.dst + off) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
       *(uint4 *)(a.dst + a.plane_bytes + off) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
     } else {
-      const long long row = row0 + r;
-      const int ex = row < a.R ? a.row_exp[row] : 0;
-      uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int e = 0; e < 16; e++) w[e >> 2] |= e4m3_rne(ldexpf(t[e], ex)) << (8 * (e & 3));
-      *(uint4 *)(a.dst + off) = make_uint4(w[0], w[1], w[2], w[3]);
+      fp8_block_unit(a, rt, kf, lane, t, a.dst + off);
     }
   }
 }
@@ -472,80 +475,6 @@ __global__ void k_pack_digits(PackArgs a) {
   pack_digits_unit<S, P>(a, blockIdx.x * 256LL + threadIdx.x);
 }
 
-// ---- FP8 scales: row maxima of |x| as float bits, then exponents ----------
-
-// one block per row (rows with unit k-stride read coalesced)
-__global__ void k_rowmax_rowwise(const float *src, long long rs, long long ks, int R, int K, uint32_t *bits) {
-  int row = blockIdx.x;
-  float m = 0.f;
-  for (int k = threadIdx.x; k < K; k += 256) m = fmaxf(m, fabsf(src[row * rs + (long long)k * ks]));
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-  __shared__ float red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) bits[row] = __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
-}
-
-// 256 consecutive rows per block, a k-slice per blockIdx.y (rows with unit
-// row-stride read coalesced); combined with atomicMax on the float bits
-__global__ void k_rowmax_colwise(const float *src, long long rs, long long ks, int R, int K, int kslice,
-                                 uint32_t *bits) {
-  int row = blockIdx.x * 256 + threadIdx.x;
-  if (row >= R) return;
-  int k0 = blockIdx.y * kslice, k1 = min(K, k0 + kslice);
-  float m = 0.f;
-  for (int k = k0; k < k1; k++) m = fmaxf(m, fabsf(src[row * rs + (long long)k * ks]));
-  atomicMax(&bits[row], __float_as_uint(m));
-}
-
-// Same maxima with 16-byte loads (the launcher checks alignment): one block
-// per row for k-contiguous rows; 1024 rows x 128 k per block for
-// row-contiguous sources, combined with atomicMax.
-__global__ void __launch_bounds__(256) k_rowmax_rowwise_v4(const float *src, long long rs, int K, uint32_t *bits) {
-  const int row = blockIdx.x;
-  const float4 *p = (const float4 *)(src + row * rs);
-  const int K4 = K >> 2;
-  float m = 0.f;
-  for (int k = threadIdx.x; k < K4; k += 256) {
-    const float4 v = p[k];
-    m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
-  }
-  for (int k = (K4 << 2) + threadIdx.x; k < K; k += 256) m = fmaxf(m, fabsf(src[row * rs + k]));
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-  __shared__ float red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) bits[row] = __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
-}
-
-__global__ void __launch_bounds__(256) k_rowmax_colwise_v4(const float *src, long long ks, int R, int K,
-                                                           uint32_t *bits) {
-  constexpr int KS = 128;
-  const int r0 = blockIdx.x * 1024 + threadIdx.x * 4;
-  const int k0 = blockIdx.y * KS, k1 = min(K, k0 + KS);
-  if (r0 >= R) return;
-  float m[4] = {0.f, 0.f, 0.f, 0.f};
-  if (r0 + 3 < R) {
-#pragma unroll 8
-    for (int k = k0; k < k1; k++) {
-      const float4 v = *(const float4 *)(src + (long long)k * ks + r0);
-      m[0] = fmaxf(m[0], fabsf(v.x));
-      m[1] = fmaxf(m[1], fabsf(v.y));
-      m[2] = fmaxf(m[2], fabsf(v.z));
-      m[3] = fmaxf(m[3], fabsf(v.w));
-    }
-  } else {
-    for (int k = k0; k < k1; k++)
-      for (int e = 0; e < 4 && r0 + e < R; e++) m[e] = fmaxf(m[e], fabsf(src[(long long)k * ks + r0 + e]));
-  }
-  for (int e = 0; e < 4 && r0 + e < R; e++) atomicMax(&bits[r0 + e], __float_as_uint(m[e]));
-}
-
-__global__ void k_exps(const uint32_t *bits, int *ex, int n) {
-  int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) ex[i] = fp8_scale_exp(__uint_as_float(bits[i]));
-}
-
 // ===========================================================================
 // the packed-fragment MFMA GEMM
 //
@@ -611,7 +540,14 @@ struct TileShape {
   static constexpr int KFC = KC / KF;
   static constexpr int P = (MODE == MODE_SPLIT3) ? 2 : 1;
   static constexpr int FRC = P * (BM / T) * KFC + P * (BN / T) * KFC;
-  static constexpr int LDS_VEC = NS * FRC * 64; // u32x4 elements of LDS
+  // block-scaled operands: one scale byte per row per 32 k of each operand,
+  // copied 256 bytes per wave instruction, every wave the same number of
+  // copies (the area is rounded up to whole rounds of copies)
+  static constexpr int NW = WGM * WGN;
+  static constexpr int SCR = IsScaled<MF>::value ? (BM / T + BN / T) * KC : 0;
+  static constexpr int SCB = (SCR / 256 + NW - 1) / NW * NW * 256;
+  static constexpr int SV = FRC * 64 + SCB / 16;  // u32x4 elements per stage
+  static constexpr int LDS_VEC = NS * SV;         // u32x4 elements of LDS
 };
 
 // One BM x BN output tile of one problem: k-chunks [c0, c1) of slice `slice`
@@ -626,7 +562,20 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   constexpr int P = (MODE == MODE_SPLIT3) ? 2 : 1;
   constexpr int RA = BM / T, RB = BN / T;
   constexpr int A_FR = P * RA * KFC, B_FR = P * RB * KFC, FRC = A_FR + B_FR;
-  constexpr int LA = A_FR / NW, LB = B_FR / NW, LPW = LA + LB;
+  // Block-scaled families (FP8): each stage also holds the chunk's scale
+  // bytes, A's [row tile][KC / 64 pairs][64] then B's likewise (SCA + SCB
+  // bytes after the fragments).  A wave copies them 4 bytes per lane (global
+  // loads into LDS place lane l's dword at +4l), 256 bytes per instruction,
+  // LS instructions per wave; when the SCT / 256 copies do not split evenly
+  // over the waves, the last round's spare copies repeat scale bytes into
+  // the padding after SCT, where nobody reads them.
+  constexpr bool SC = IsScaled<MF>::value;
+  constexpr int SCA = SC ? RA * KC : 0, SCB = SC ? RB * KC : 0, SCT = SCA + SCB;
+  static_assert(SCT % 256 == 0, "scale bytes per chunk must be whole 256-byte copies");
+  constexpr int LS = (SCT / 256 + NW - 1) / NW;
+  constexpr int SV = TileShape<MF, BM, BN, KC, WGM, WGN, MODE, NS>::SV; // u32x4 per stage
+  static_assert(SV == FRC * 64 + LS * NW * 256 / 16, "stage layout");
+  constexpr int LA = A_FR / NW, LB = B_FR / NW, LPW = LA + LB + LS;
   static_assert(A_FR % NW == 0 && B_FR % NW == 0, "fragments must split evenly over waves");
   static_assert(KFC % FR == 0 && TM >= 1 && TN >= 1, "bad tile");
   static_assert(NS >= 2 && LPW * (NS - 1) <= 63, "too many loads in flight for vmcnt");
@@ -696,6 +645,23 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     }
   }
 
+  // scale copies: lane byte o of copy s of the chunk's scale area reads row
+  // tile o / KC of A (or, past SCA, of B), bytes (o % KC) of the chunk's KC
+  // (the chunk's pairs are contiguous in each row tile's scales); spare
+  // copies (o >= SCT) read the bytes of o % SCT again
+  const unsigned char *sdir[SC ? LS : 1];
+  if constexpr (SC) {
+    const long long pairs = g.nKF >> 1;
+#pragma unroll
+    for (int i = 0; i < LS; i++) {
+      const int o = ((wave + i * NW) * 256 + lane * 4) % SCT;
+      if (o < SCA)
+        sdir[i] = g.a_sc + ((long long)(bm * RA + o / KC) * pairs) * 64 + o % KC;
+      else
+        sdir[i] = g.b_sc + ((long long)(bn * RB + (o - SCA) / KC) * pairs) * 64 + (o - SCA) % KC;
+    }
+  }
+
   Acc acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; i++)
@@ -711,7 +677,7 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
     const long long ko = (long long)kc * KFC * NUMC_FRAG_BYTES;
     const unsigned char *pA = g.A + ko + (long long)(pa0 + q) * g.a_plane + loff;
     const unsigned char *pB = g.B + ko + (long long)(pb0 - q) * g.b_plane + loff;
-    u32x4 *base = lds + s * FRC * 64;
+    u32x4 *base = lds + s * SV;
     const long long kb = (long long)kc * KFC * 32; // in place: 32 bytes of each row per k-fragment
     if constexpr (ADIR) {
 #pragma unroll
@@ -735,11 +701,27 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
         __builtin_amdgcn_global_load_lds((global_void_t *)(pB + uoffB[i]),
                                          (lds_void_t *)(base + (A_FR + wave + i * NW) * 64), 16, 0, 0);
     }
+    if constexpr (SC) {
+      unsigned char *sbase = (unsigned char *)(base + FRC * 64);
+#pragma unroll
+      for (int i = 0; i < LS; i++)
+        __builtin_amdgcn_global_load_lds((global_void_t *)(sdir[i] + (long long)kc * KC),
+                                         (lds_void_t *)(sbase + (wave + i * NW) * 256), 4, 0, 0);
+    }
   };
-  // fragments of one k step, double buffered in registers
+  // fragments of one k step, double buffered in registers (with their
+  // scale bytes: lane l's byte of pair kf / 2 in each row tile)
   u32x4 af[2][P][TM][FR], bfr[2][P][TN][FR];
+  int sca[2][SC ? TM : 1], scb[2][SC ? TN : 1];
   auto fload = [&](int s, int kf, int buf) __attribute__((always_inline)) {
-    const u32x4 *base = lds + s * FRC * 64;
+    const u32x4 *base = lds + s * SV;
+    if constexpr (SC) {
+      const unsigned char *sl = (const unsigned char *)(base + FRC * 64) + (kf >> 1) * 64 + lane;
+#pragma unroll
+      for (int i = 0; i < TM; i++) sca[buf][i] = sl[(wm * TM + i) * KC];
+#pragma unroll
+      for (int j = 0; j < TN; j++) scb[buf][j] = sl[SCA + (wn * TN + j) * KC];
+    }
 #pragma unroll
     for (int p = 0; p < P; p++)
 #pragma unroll
@@ -775,6 +757,11 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
           MF::mma(acc[i][j], af[buf][0][i], bfr[buf][1][j]);
           MF::mma(acc[i][j], af[buf][0][i], bfr[buf][0][j]);
         }
+    } else if constexpr (SC) {
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) MF::mma(acc[i][j], af[buf][0][i], bfr[buf][0][j], sca[buf][i], scb[buf][j]);
     } else {
 #pragma unroll
       for (int i = 0; i < TM; i++)
@@ -794,7 +781,7 @@ __device__ __attribute__((always_inline)) inline void gemm_tile(const GemmArgs &
   // being waited for, every wait takes a constant count and the steady-state
   // loop has no branches.
   constexpr int NKS = KFC / FR;
-  constexpr int NRD = P * (TM + TN) * FR;                                 // LDS reads per k step
+  constexpr int NRD = P * (TM + TN) * FR + (SC ? TM + TN : 0);            // LDS reads per k step
   constexpr int NMF = TM * TN * (MODE == MODE_SPLIT3 ? 3 : 1) * MF::NMMA; // MFMAs per k step
   for (int pass = npass - 1; pass >= 0; pass--) {
     if constexpr (FUSE) {
@@ -1127,13 +1114,13 @@ static const GemmConfig kSplit[] = {
     CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 8, 4),
 };
 static const GemmConfig kFP8[] = {
-    CFGS(MfFP8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
-    CFGS(MfFP8, 256, 256, 64, 4, 2, MODE_SINGLE, EPI_FP8, 1, 4),
-    CFGS(MfFP8, 256, 256, 128, 4, 2, MODE_SINGLE, EPI_FP8, 1, 2),
-    CFGS(MfFP8, 256, 256, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
-    CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
-    CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 4),
-    CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_FP8, 4, 4),
+    CFGS(MfFP8, 256, 128, 64, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfFP8, 256, 256, 64, 4, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfFP8, 256, 256, 128, 4, 2, MODE_SINGLE, EPI_F32, 1, 2),
+    CFGS(MfFP8, 256, 256, 64, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_F32, 1, 4),
+    CFGS(MfFP8, 128, 128, 128, 2, 2, MODE_SINGLE, EPI_F32, 4, 4),
 };
 static const GemmConfig kF64[] = {
     CFGB(MfF64, 128, 128, 16, 2, 2, EPI_F64, 1, 4),
@@ -1181,7 +1168,7 @@ static const GemmConfig kF16[] = {
 static const GemmConfig kStridedBatch[NUMC_FAMILY_COUNT][2] = {
     {CFGD(MfF32, 128, 128, 16, 2, 2, EPI_F32, 1, 2), CFGD(MfF32, 64, 64, 32, 2, 2, EPI_F32, 1, 2)},
     {CFGS(MfBF16, 128, 128, 16, 2, 2, MODE_SPLIT3, EPI_F32, 1, 2), CFGS(MfBF16, 64, 64, 32, 2, 2, MODE_SPLIT3, EPI_F32, 1, 2)},
-    {CFGS(MfFP8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_FP8, 1, 2), CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_FP8, 1, 2)},
+    {CFGS(MfFP8, 128, 128, 64, 2, 2, MODE_SINGLE, EPI_F32, 1, 2), CFGS(MfFP8, 64, 64, 128, 2, 2, MODE_SINGLE, EPI_F32, 1, 2)},
     {CFGB(MfF64, 64, 64, 16, 2, 2, EPI_F64, 1, 2), CFGB(MfF64, 64, 64, 32, 2, 2, EPI_F64, 1, 2)},
     {CFGDA(MfI8, 128, 128, 64, 2, 2, EPI_INT, 1, 2), CFGDA(MfI8, 64, 64, 64, 2, 2, EPI_INT, 1, 2)},
     {CFGB(MfBF16, 128, 128, 32, 2, 2, EPI_BF16, 1, 2), CFGB(MfBF16, 64, 64, 64, 2, 2, EPI_BF16, 1, 2)},
@@ -1909,28 +1896,6 @@ static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, u
   }
 }
 
-static void launch_rowmax(const float *src, long long rs, long long ks, int R, int K, uint32_t *bits,
-                          hipStream_t st) {
-  if (R <= 0) return;
-  const bool al = (((uintptr_t)src) & 15) == 0;
-  if (ks == 1 && al && (rs & 3) == 0) {
-    hipLaunchKernelGGL(k_rowmax_rowwise_v4, dim3(R), dim3(256), 0, st, src, rs, K, bits);
-    return;
-  }
-  if (rs == 1 && al && (ks & 3) == 0) {
-    dim3 grid((unsigned)cdiv(R, 1024), (unsigned)cdiv(K > 0 ? K : 1, 128));
-    hipLaunchKernelGGL(k_rowmax_colwise_v4, grid, dim3(256), 0, st, src, ks, R, K, bits);
-    return;
-  }
-  if (ks == 1 || (rs != 1 && K >= 256)) {
-    hipLaunchKernelGGL(k_rowmax_rowwise, dim3(R), dim3(256), 0, st, src, rs, ks, R, K, bits);
-  } else {
-    int kslice = 256;
-    dim3 grid(blocks_for(R), (unsigned)cdiv(K > 0 ? K : 1, kslice));
-    hipLaunchKernelGGL(k_rowmax_colwise, grid, dim3(256), 0, st, src, rs, ks, R, K, kslice, bits);
-  }
-}
-
 int numc_problem_planes(const GemmProblem &p) {
   return p.family == NUMC_FAMILY_I8 ? digit_planes(p.in_dt, p.out_dt) : 1;
 }
@@ -1953,8 +1918,9 @@ size_t numc_gemm_workspace(const GemmProblem &p, const GemmConfig &c, size_t *au
   size_t raw = (p.family == NUMC_FAMILY_F64) ? 8 : 4;
   size_t partial = c.cfg.split_k > 1 ? (size_t)c.cfg.split_k * p.M * p.N * raw : 0;
   const size_t nz = p.batch > 1 ? (size_t)p.batch : 1;
-  size_t bytes = (size_t)(pa + pb) * planes * nz + partial + 3 * 256;
-  if (aux_bytes) *aux_bytes = 2 * NUMC_FLAG_INTS * sizeof(int) + (size_t)(p.M + p.N) * 8 + 512;
+  const size_t scales = p.family == NUMC_FAMILY_FP8 ? (size_t)(pa + pb) / 32 : 0; // FP8 block scales
+  size_t bytes = (size_t)(pa + pb) * planes * nz + scales + partial + 4 * 256;
+  if (aux_bytes) *aux_bytes = 2 * NUMC_FLAG_INTS * sizeof(int) + 512;
   return bytes;
 }
 
@@ -2023,24 +1989,19 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   unsigned char *wsb = (unsigned char *)ws;
   unsigned char *Ap = wsb;
   unsigned char *Bp = Ap + pla * planes * nz;
-  void *partial = c.cfg.split_k > 1 ? (void *)(Bp + plb * planes) : nullptr;
+  // FP8: the block scales (a byte per row of each packed fragment) follow the
+  // planes; then the split-K partial sums
+  unsigned char *Asc = Bp + plb * planes * nz;
+  unsigned char *Bsc = Asc + (family == NUMC_FAMILY_FP8 ? pla / 32 : 0);
+  unsigned char *after = Bsc + (family == NUMC_FAMILY_FP8 ? plb / 32 : 0);
+  after = (unsigned char *)(((uintptr_t)after + 255) & ~(uintptr_t)255);
+  void *partial = c.cfg.split_k > 1 ? (void *)after : nullptr;
   const size_t ein = numc_dtype_size(p.in_dt), eout = numc_dtype_size(p.out_dt);
 
-  // aux: the digit plane masks (NUMC_FLAG_INTS ints per operand), then FP8
-  // row max bits / exponents
+  // aux: the digit plane masks (NUMC_FLAG_INTS ints per operand)
   int *flags = (int *)aux;
-  uint32_t *bitsA = (uint32_t *)((unsigned char *)aux + 2 * NUMC_FLAG_INTS * sizeof(int));
-  uint32_t *bitsB = bitsA + p.M;
-  int *expA = (int *)(bitsB + p.N);
-  int *expB = expA + p.M;
 
   PackArgs pk{};
-  if (family == NUMC_FAMILY_FP8) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(p.M + p.N)), dim3(256), 0, st, bitsA, (long long)(p.M + p.N), 0u);
-    launch_rowmax((const float *)p.A, p.rsa, p.csa, p.M, p.K, bitsA, st);
-    launch_rowmax((const float *)p.B, p.csb, p.rsb, p.N, p.K, bitsB, st);
-    hipLaunchKernelGGL(k_exps, dim3(blocks_for(p.M + p.N)), dim3(256), 0, st, bitsA, expA, p.M + p.N);
-  }
   // one pack launch for both operands where the batched walks cover the
   // family.  A single int8 digit plane needs no plane masks (they only let
   // the GEMM skip empty plane products), so that case sets none.
@@ -2070,13 +2031,13 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   // A: rows M, element (i, k) at A[i*rsa + k*csa]
   pk.src = p.A; pk.rs = p.rsa; pk.ks = p.csa; pk.R = p.M; pk.K = p.K;
   pk.nRT = nRTa; pk.nKF = nKF; pk.dst = Ap; pk.plane_bytes = pla;
-  pk.row_exp = expA; pk.flags = masks ? flags : nullptr;
+  pk.sc = Asc; pk.flags = masks ? flags : nullptr;
   pk.src_zs = p.a_bs * (long long)ein; pk.dst_zs = pla * planes;
   // B^T: rows N, element (j, k) at B[k*rsb + j*csb]
   PackArgs pb = pk;
   pb.src = p.B; pb.rs = p.csb; pb.ks = p.rsb; pb.R = p.N; pb.K = p.K;
   pb.nRT = nRTb; pb.nKF = nKF; pb.dst = Bp; pb.plane_bytes = plb;
-  pb.row_exp = expB; pb.flags = masks ? flags + NUMC_FLAG_INTS : nullptr;
+  pb.sc = Bsc; pb.flags = masks ? flags + NUMC_FLAG_INTS : nullptr;
   pb.src_zs = p.b_bs * (long long)ein; pb.dst_zs = plb * planes;
   // (each pack launch is one profiled pack)
   if (adir && bdir) {
@@ -2133,8 +2094,8 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
   g.rso = p.rso;
   g.cso = p.cso;
   g.out_bytes = (int)numc_dtype_size(p.out_dt);
-  g.row_exp = expA;
-  g.col_exp = expB;
+  g.a_sc = Asc;
+  g.b_sc = Bsc;
   g.partial = partial;
   g.nprod = 1;
   g.pa[0] = g.pb[0] = 0;
@@ -2167,7 +2128,7 @@ int numc_gemm_launch(NumcCtx *ctx, const GemmProblem &p, const GemmConfig &c, vo
       if (bias) hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64B>), dim3(nb), dim3(256), 0, st, g);
       else hipLaunchKernelGGL((k_splitk_reduce<double, EPI_F64>), dim3(nb), dim3(256), 0, st, g);
       break;
-    case NUMC_FAMILY_FP8: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_FP8>), dim3(nb), dim3(256), 0, st, g); break;
+    case NUMC_FAMILY_FP8: hipLaunchKernelGGL((k_splitk_reduce<float, EPI_F32>), dim3(nb), dim3(256), 0, st, g); break;
     case NUMC_FAMILY_BF16:
       if (bias) hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16B>), dim3(nb), dim3(256), 0, st, g);
       else hipLaunchKernelGGL((k_splitk_reduce<float, EPI_BF16>), dim3(nb), dim3(256), 0, st, g);

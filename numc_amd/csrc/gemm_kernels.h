@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -85,14 +86,22 @@ struct MfI8 { // v_mfma_i32_32x32x32_i8
   __device__ static inline int col(int lane) { return lane & 31; }
 };
 
-struct MfFP8 { // v_mfma_scale_f32_32x32x64_f8f6f4, both operands OCP e4m3, unit scales
+// v_mfma_scale_f32_32x32x64_f8f6f4, both operands OCP e4m3 with E8M0 block
+// scales (2^(s-127)).  The MFMA's 64 k are two packed fragments: a[0] holds
+// k 0-31 (lane half h: k 16h..16h+15) and a[1] k 32-63.  Measured on gfx950:
+// the scale in lane r (r < 32) multiplies k 0-31 of row r -- fragment a[0],
+// bytes of both lane halves -- and the scale in lane r + 32 multiplies
+// k 32-63 -- fragment a[1]; the same for the columns of B.  So a scale
+// covers one row of one packed fragment: 32 consecutive k, 32-aligned.
+struct MfFP8 {
   static constexpr int T = 32, KF = 32, FR = 2, NACC = 16, NMMA = 1;
+  static constexpr bool SCALED = true;
   using Acc = f32x16;
   using Raw = float;
-  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b) {
+  __device__ static inline void mma(Acc &c, const u32x4 *a, const u32x4 *b, int sa, int sb) {
     i32x8 av = __builtin_shufflevector(__builtin_bit_cast(i32x4, a[0]), __builtin_bit_cast(i32x4, a[1]), 0, 1, 2, 3, 4, 5, 6, 7);
     i32x8 bv = __builtin_shufflevector(__builtin_bit_cast(i32x4, b[0]), __builtin_bit_cast(i32x4, b[1]), 0, 1, 2, 3, 4, 5, 6, 7);
-    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, 127, 0, 127);
+    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, sa, 0, sb);
   }
   __device__ static inline int row(int v, int lane) { return 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
   __device__ static inline int col(int lane) { return lane & 31; }
@@ -118,11 +127,15 @@ struct MfF64 { // v_mfma_f64_16x16x4_f64 x2 per fragment
 // ---------------------------------------------------------------------------
 
 enum GemmMode { MODE_SINGLE = 0, MODE_SPLIT3 = 1 };
+
+// MFMA families whose operands carry per-block scales (MF::SCALED)
+template <class MF, class = void> struct IsScaled { static constexpr bool value = false; };
+template <class MF> struct IsScaled<MF, std::void_t<decltype(MF::SCALED)>> { static constexpr bool value = MF::SCALED; };
 // EPI_BF16 / EPI_F16: FP32 sums rounded to nearest even into 16-bit floats.
 // EPI_*B (= EPI_* + 4): the same outputs plus a per-column bias (numc_linear),
 // separate kernels so that the plain stores keep their code and register budget.
 enum GemmEpi {
-  EPI_F32 = 0, EPI_F64 = 1, EPI_FP8 = 2, EPI_INT = 3,
+  EPI_F32 = 0, EPI_F64 = 1, EPI_INT = 3,
   EPI_F32B = 4, EPI_F64B = 5, EPI_INTB = 7,
   EPI_BF16 = 8, EPI_F16 = 9, EPI_BF16B = 12, EPI_F16B = 13
 };
@@ -155,7 +168,7 @@ struct GemmArgs {
   int shift_pass;             // integer digit pass s (adds (sum << 8s))
   int passes;                 // > 1: every digit pass in this launch (int8 MFMA kernels), shift_pass 0
   const int *plane_flags;     // integer: [0] nonzero-plane mask of A, [1] of B (null: use pa/pb)
-  const int *row_exp, *col_exp; // FP8 power-of-two scale exponents
+  const unsigned char *a_sc, *b_sc; // FP8: E8M0 block scales, [row tile][k-fragment pair][64 lanes]
   void *partial;              // split-K partial sums [ksplit][M][N] (raw accumulator type)
   int nprod;                  // MODE_SINGLE: products concatenated along K
   int pa[4], pb[4];           // plane of A / B for each product
@@ -250,8 +263,6 @@ __device__ inline void epi_store(const GemmArgs &g, int row, int col, Raw v, typ
     ((double *)g.C)[off] = (double)v;
   } else if constexpr (EPI == EPI_F64B) {
     ((double *)g.C)[off] = (double)v + bias;
-  } else if constexpr (EPI == EPI_FP8) {
-    ((float *)g.C)[off] = ldexpf((float)v, -(g.row_exp[row] + g.col_exp[col]));
   } else if constexpr (EPI == EPI_BF16 || EPI == EPI_BF16B || EPI == EPI_F16 || EPI == EPI_F16B) {
     float r = (float)v;
     if constexpr (EPI == EPI_BF16B || EPI == EPI_F16B) r += bias;

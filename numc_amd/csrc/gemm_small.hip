@@ -402,8 +402,8 @@ const GemmConfig kDirF16[] = {DCFGB(MfF16, 1, 1, 4, 4, EPI_F16), DCFGB(MfF16, 2,
 #undef DCFGB
 } // namespace
 
-// the direct configurations of a family (none for FP8, whose per-row scales
-// need a pass over the operands first)
+// the direct configurations of a family (none for FP8, whose operands are
+// quantised to e4m3 with their block scales by the pack)
 const GemmConfig *numc_direct_configs(int family, int *count) {
   const GemmConfig *t = nullptr;
   int n = 0;

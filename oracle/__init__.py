@@ -186,20 +186,38 @@ def _e4m3_table():
     return np.array([lib().orc_e4m3_to_f32(q) for q in range(256)], np.float64)
 
 
-def fp8_quantize_rows(x):
-    """Rows of x scaled by 2^e (e = fp8_scale_exp of the row's max |x|), rounded
-    to e4m3, decoded and unscaled back to float64: the exact operand rows the
-    FP8 method multiplies.  Columns of B are the rows of B^T."""
+def fp8_scale_exps(m):
+    """fp8_scale_exp over an array of maxima (numpy; the tests check it against
+    the C function element by element)"""
+    m = np.asarray(m, np.float32)
+    _, q = np.frexp(m)                      # m = f * 2^q, f in [0.5, 1)
+    e = np.clip(8 - q.astype(np.int64), -126, 126)
+    return np.where(m > 0, e, 0)
+
+
+def fp8_quantize_blocks(x):
+    """Each row of x in blocks of 32 consecutive k (block b holds k 32b..32b+31):
+    the block scaled by 2^e, e = fp8_scale_exp of the block's max |x|, rounded
+    to e4m3, decoded and unscaled back to float64 -- the exact operand rows the
+    FP8 method multiplies (every row of every 32-k packed fragment carries its
+    own power-of-two scale).  Columns of B are the rows of B^T."""
     x = np.asarray(x, np.float32)
-    e = np.array([fp8_scale_exp(np.max(np.abs(r)) if r.size else 0.0) for r in x], np.int64)
-    q = _e4m3_table()[e4m3_array(np.ldexp(x, e[:, None]).astype(np.float32))]
-    return np.ldexp(q, -e[:, None])
+    R, K = x.shape
+    nb = -(-K // 32)
+    xp = np.zeros((R, nb * 32), np.float32)
+    xp[:, :K] = x
+    blocks = xp.reshape(R, nb, 32)
+    e = fp8_scale_exps(np.max(np.abs(blocks), axis=2, initial=0.0))      # (R, nb)
+    scaled = np.ldexp(blocks.astype(np.float64), e[:, :, None]).astype(np.float32)
+    q = _e4m3_table()[e4m3_array(scaled)]
+    return np.ldexp(q, -e[:, :, None]).reshape(R, nb * 32)[:, :K]
 
 
 def fp8_quantized_inputs(a, b):
-    """Per-row scaled e4m3 A, per-column scaled e4m3 B, decoded and unscaled back to
-    float64: the exact operands the FP8 method multiplies."""
-    return fp8_quantize_rows(a), fp8_quantize_rows(np.asarray(b).T).T
+    """A with a scale per row per 32 k, B with a scale per column per 32 k, e4m3,
+    decoded and unscaled back to float64: the exact operands the FP8 method
+    multiplies."""
+    return fp8_quantize_blocks(a), fp8_quantize_blocks(np.asarray(b).T).T
 
 
 # ---- GEMM consumers of numc's roadmap (ROADMAP.md:139-140) --------------------

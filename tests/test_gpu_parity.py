@@ -190,6 +190,29 @@ def test_fp8_spread_magnitudes(ctx):
         assert err <= -(-K // 64) * 2.0 ** -11 + (K + 2) * U, (M, K, N, err)
 
 
+def test_fp8_block_scales(ctx):
+    """Magnitudes that change by 2^40 from one 32-k block to the next along each
+    row of A and column of B, so that every output sums products from tiny and
+    from huge blocks in equal measure: one scale per row (or column) would
+    flush the tiny blocks to zero, a scale per 32-k block keeps them.  Checked
+    against the quantised operands and end to end against the FP64 sums of the
+    unquantised inputs (e4m3 keeps 3 mantissa bits: 2 * 2^-4 per product)."""
+    rng = np.random.default_rng(320)
+    for M, K, N in [(64, 128, 96), (200, 320, 130), (96, 1000, 64)]:
+        blk = np.arange(K) // 32
+        sa = np.where(blk % 2 == 0, -40.0, 0.0)     # A: even blocks tiny
+        sb = np.where(blk % 2 == 0, 40.0, 0.0)      # B: even blocks huge
+        a = np.ldexp(rng.uniform(-1, 1, (M, K)), sa[None, :].astype(int)).astype(np.float32)
+        b = np.ldexp(rng.uniform(-1, 1, (K, N)), sb[:, None].astype(int)).astype(np.float32)
+        got = gemm(ctx, a, b, method=nc.METHOD_FP8_SCALED)
+        qa, qb = oracle.fp8_quantized_inputs(a, b)
+        ref, absr = oracle.fref(qa, qb)
+        err = oracle.err_metrics(got, ref, absr)[0]
+        assert err <= -(-K // 64) * 2.0 ** -11 + (K + 2) * U, (M, K, N, err)
+        err0 = cw_err(got, a, b)
+        assert err0 <= 2 * 2.0 ** -4 + 2.0 ** -8, (M, K, N, err0)
+
+
 @pytest.mark.parametrize("cfg", range(nc.config_count(nc.FAMILY_F64)))
 def test_f64_configs(ctx, cfg):
     rng = np.random.default_rng(400 + cfg)

@@ -143,8 +143,9 @@ def test_config3_fp8_8192(ill):
         nc.check(nc.matmul(A, B, O, nc.METHOD_FP8_SCALED), "fp8 matmul")
         c = O.numpy()
     rows, cols, ri, ci = _sample(50 + ill)
-    qa = np.ascontiguousarray(oracle.fp8_quantize_rows(a[rows]))          # full rows: their own scales
-    qb = np.ascontiguousarray(oracle.fp8_quantize_rows(b[:, cols].T).T)   # full columns
+    # whole rows of A and columns of B: each 32-k block carries its own scale
+    qa = np.ascontiguousarray(oracle.fp8_quantize_blocks(a[rows]))
+    qb = np.ascontiguousarray(oracle.fp8_quantize_blocks(b[:, cols].T).T)
     ref, absr = oracle.fref(qa, qb, ri, ci)
     got = c[rows[ri], cols[ci]].astype(np.float64)
     err = float(np.max(np.abs(got - ref) / absr))

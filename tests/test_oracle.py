@@ -12,7 +12,9 @@ Parity status of each oracle function and what pins it:
   orc_bf16_rne / split  torch's bfloat16 conversion (independent RNE), the
                       2^-17 representation bound, exhaustive exponent sweep.
   orc_e4m3_rne / to_f32 torch.float8_e4m3fn for every code and for RNE ties.
-  orc_fp8_scale_exp   closed form: scaled maximum lies in [128, 256).
+  orc_fp8_scale_exp   closed form: scaled maximum lies in [128, 256); the
+                      numpy fp8_scale_exps and fp8_quantize_blocks element by
+                      element against it and the scalar e4m3 encoder.
   generator           splitmix64 / xoshiro256** published first outputs.
 No oracle function is left unpinned.
 """
@@ -182,23 +184,48 @@ def test_e4m3_against_torch():
     assert oracle.e4m3(np.float32([500.0, -1e30]))[0] == 0x7E
 
 
-def test_e4m3_array_equals_scalar_and_quantized_rows():
+def test_e4m3_array_equals_scalar_and_quantized_blocks():
     """the array encoder is the scalar one element by element (every finite
-    bf16-representable magnitude plus random values), and the row quantizer
-    scales each row's maximum into [128, 256) before rounding"""
+    bf16-representable magnitude plus random values), and the block quantizer
+    scales each 32-k block's maximum into [128, 256) before rounding, block by
+    block, with the last block of a ragged row short"""
     hi = np.arange(0, 1 << 16, dtype=np.uint32)
     x = np.concatenate([(hi << 16).view(np.float32),
                         np.random.default_rng(2).standard_normal(5000).astype(np.float32)])
     x = x[np.isfinite(x)]
     assert np.array_equal(oracle.e4m3_array(x), oracle.e4m3(x))
     rng = np.random.default_rng(3)
-    a = (rng.standard_normal((7, 33)) * 2.0 ** rng.integers(-20, 20, (7, 1))).astype(np.float32)
-    q = oracle.fp8_quantize_rows(a)
+    a = (rng.standard_normal((7, 77)) * 2.0 ** rng.integers(-20, 20, (7, 77))).astype(np.float32)
+    a[2, 32:64] = 0.0                       # an all-zero block: scale 2^0
+    q = oracle.fp8_quantize_blocks(a)
+    assert q.shape == a.shape
     for r in range(a.shape[0]):
-        e = oracle.fp8_scale_exp(np.max(np.abs(a[r])))
-        assert 128.0 <= np.max(np.abs(a[r])) * 2.0 ** e < 256.0
-        want = oracle.e4m3_decode(oracle.e4m3(np.ldexp(a[r], e).astype(np.float32))).astype(np.float64)
-        assert np.array_equal(q[r], np.ldexp(want, -e))
+        for b0 in range(0, a.shape[1], 32):
+            blk = a[r, b0:b0 + 32]
+            m = np.max(np.abs(blk))
+            e = oracle.fp8_scale_exp(m)
+            if m > 0:
+                assert 128.0 <= m * 2.0 ** e < 256.0
+            else:
+                assert e == 0
+            want = oracle.e4m3_decode(oracle.e4m3(np.ldexp(blk, e).astype(np.float32))).astype(np.float64)
+            assert np.array_equal(q[r, b0:b0 + 32], np.ldexp(want, -e))
+    # columns of B are quantized as the rows of B^T
+    qa, qb = oracle.fp8_quantized_inputs(a[:, :40], a[:, :40].T)
+    assert np.array_equal(qb, qa.T)
+
+
+def test_fp8_scale_exps_equal_scalar():
+    """the numpy block exponents are the C function's on maxima across the whole
+    float32 range (zero, subnormals, both clamps, powers of two and neighbours)"""
+    m = np.concatenate([np.float32([0.0, 1e-45, 1e-40, 1.1754944e-38, 2.0 ** -126, 1e-30, 0.5, 1.0, 448.0,
+                                    3e38, np.finfo(np.float32).max]),
+                        np.float32(2.0) ** np.arange(-149, 128).astype(np.float32),
+                        np.nextafter(np.float32(2.0) ** np.arange(-100, 100).astype(np.float32), np.float32(0)),
+                        np.abs(np.random.default_rng(4).standard_normal(2000).astype(np.float32)) *
+                        np.float32(2.0) ** np.random.default_rng(5).integers(-60, 60, 2000).astype(np.float32)])
+    want = np.array([oracle.fp8_scale_exp(float(v)) for v in m])
+    assert np.array_equal(oracle.fp8_scale_exps(m), want)
 
 
 def test_fp8_scale_exp_closed_form():
