@@ -3,8 +3,9 @@
 Workload (the headline metric's configuration): A, B ~ N(0,1) float32 from the
 documented seeded generator, resident in HBM before timing.  One step = one
 complete GEMM call (pack kernels + MFMA kernel) of the split-BF16 method;
-the plain FP32 and the per-row/column-scaled FP8 methods are timed the same
-way and reported beside it.  value = 2*M*N*K / ms_per_step (FP32-equivalent
+the plain FP32 and the block-scaled FP8 (e4m3, a power-of-two scale per 32 k
+of each row of A and column of B) methods are timed the same way and
+reported beside it.  value = 2*M*N*K / ms_per_step (FP32-equivalent
 TFLOP/s; extra MFMA products are not counted as work).
 
 Configuration: the one numc_matmul runs for the shape (the committed
@@ -26,8 +27,10 @@ AVX2 + OpenMP) on the same bytes, min of 2 after a warmup, on this host.
 
 configs: the other workloads, each with its own roofline and cpu_baseline --
 float32 512^3, float64 2048^3, int32 2048^3 and int8 4096^3 through
-numc_matmul (whole call, plus the library's per-launch kernel times), and the
-uneven batch of 4096 problems (M, N, K log-uniform in [16, 1024]) through
+numc_matmul (whole call, plus the library's per-launch kernel times), one
+GPU's share of an 8-way row split of 8192^3 (1024 x 8192 x 8192, each
+method), and the uneven batch of 4096 problems (M, N, K log-uniform in
+[16, 1024]) through
 numc_matmul_batched in float32, split-BF16 and int8 -> int32, after
 numc_matmul_batched_tune picks its schedule.  host_cpu names the host's CPU
 model, cores and ISA flags.  --no-configs leaves them out; --only configs or
@@ -289,6 +292,39 @@ def bench_configs(nc, steps, warmup, cores, with_cpu):
             else:
                 e["cpu_baseline"] = {"value": None, "unit": unit, "cores": cores, "kind": "reference", "sample": why}
         res[key] = e
+    return res
+
+
+def bench_shard(nc, steps, warmup, m=1024, n=8192):
+    """One GPU's share of the 8192^3 FP32 GEMM split 8 ways by rows of C:
+    A's 1024-row panel times the whole B, N(0,1) inputs, each method; whole
+    call and kernel, with the kernel's roofline (the multi-GPU exchange is not
+    part of it)."""
+    ctx = nc.Ctx()
+    ctx.set_sync(False)
+    A, B = ctx.array((m, n), nc.FLOAT32), ctx.array((n, n), nc.FLOAT32)
+    A.fill_random(nc.DIST_NORMAL, 3, 0)
+    B.fill_random(nc.DIST_NORMAL, 3, 1)
+    Cm = ctx.array((m, n), nc.FLOAT32, zeros=True)
+    for x in (A, B, Cm):
+        x.to_device()
+    work = 2.0 * m * n * n
+    res = {"workload": "FP32 GEMM %dx%dx%d (one GPU's rows of C in an 8-way row split of %d^3), N(0,1), "
+                       "operands resident in HBM" % (m, n, n, n)}
+    for name, method, fam, peak in (("split_bf16", nc.METHOD_SPLIT_BF16, nc.FAMILY_SPLIT_BF16, PEAK["split_bf16"]),
+                                    ("f32", nc.METHOD_PLAIN, nc.FAMILY_F32, PEAK["f32"]),
+                                    ("fp8", nc.METHOD_FP8_SCALED, nc.FAMILY_FP8, PEAK["fp8"])):
+        t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul(A, B, Cm, method), fam, steps, warmup)
+        cfg = nc.lib().numc_selector_choice(fam, m, n, n, 0)
+        res[name] = {"ms_per_call": round(t["ms_per_call"], 4),
+                     "tflops_whole_call": round(work / t["ms_per_call"] * 1e-9, 2),
+                     "kernel_ms": round(t["kernel_ms"], 4), "pack_ms": round(t["pack_ms"], 4),
+                     "config": nc.config_get(fam, cfg) if cfg >= 0 else None,
+                     "roofline": roofline_entry(work, float(4 * (m * n + n * n + m * n)), t["kernel_ms"], peak,
+                                                "TFLOP/s", "%s GEMM kernel (avg of %d launches)" % (name, steps))}
+        log("bench: shard %dx%dx%d %s %.4f ms/call (kernel %.4f ms, %.1f TF whole call)" %
+            (m, n, n, name, t["ms_per_call"], t["kernel_ms"], work / t["ms_per_call"] * 1e-9))
+    ctx.close()
     return res
 
 
@@ -633,8 +669,9 @@ def main():
               flush=True)
         return
     if args.only == "configs":
-        print(json.dumps({"configs": bench_configs(nc, args.steps, args.warmup, cores, not args.no_cpu),
-                          "host_cpu": host_cpu()}), flush=True)
+        configs = bench_configs(nc, args.steps, args.warmup, cores, not args.no_cpu)
+        configs["c3_shard8_rows"] = bench_shard(nc, args.steps, args.warmup, m=args.n // 8, n=args.n)
+        print(json.dumps({"configs": configs, "host_cpu": host_cpu()}), flush=True)
         return
     n = args.n
     flops = 2.0 * n * n * n
@@ -736,6 +773,8 @@ def main():
     if not args.no_configs:
         log("bench: the other configurations")
         configs = bench_configs(nc, args.steps, args.warmup, cores, not args.no_cpu)
+        log("bench: one GPU's share of an 8-way row split of %d^3" % n)
+        configs["c3_shard8_rows"] = bench_shard(nc, args.steps, args.warmup, m=n // 8, n=n)
     if not args.no_batched:
         log("bench: batched uneven GEMM, %d problems" % args.batch)
         configs["c4_batch_%d" % args.batch] = bench_batched(nc, args.batch, args.seed + 1, max(1, args.steps // 2),
