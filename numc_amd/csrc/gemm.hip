@@ -181,8 +181,8 @@ __device__ __attribute__((always_inline)) inline void fp8_block_unit(const PackA
   // v_cvt_pk_fp8_f32 rounds to nearest even into OCP e4m3.  Below 448 in
   // magnitude (a scaled block lies below 256) it matched a software RNE
   // encoder on every bf16-representable input, every e4m3 rounding tie with
-  // its float neighbours and 4M random values.  (A non-finite input makes its
-  // block's codes NaN.)
+  // its float neighbours and 4M random values.  (A non-finite input becomes a
+  // NaN code, so the output row or column it reaches is NaN.)
   uint32_t w[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
