@@ -377,7 +377,7 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
             if thresh < 0:
                 nc.check(thresh, "numc_matmul_batched_tune")
         else:   # a fixed schedule (profiling runs): the direct kernel where the family has one
-            want_direct = schedule[0] and fam in (nc.FAMILY_F32, nc.FAMILY_I8)
+            want_direct = schedule[0] and fam in (nc.FAMILY_F32, nc.FAMILY_SPLIT_BF16, nc.FAMILY_I8)
             nc.check(nc.lib().numc_matmul_batched_set_schedule(fam, int(want_direct), schedule[1]), "schedule")
             thresh = schedule[1]
         H = nc.BatchHandles(As, Bs, Os)   # the handle arrays, built once as a C caller keeps them

@@ -171,7 +171,8 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
   const int thresh = numc_batch_threshold(family);
   // the direct kernel reads one int8 digit plane: int8 inputs only
   bool direct = numc_batch_direct(family) &&
-                (family == NUMC_FAMILY_F32 || (family == NUMC_FAMILY_I8 && same_in && in_dt == NUMC_DTYPE_INT8));
+                (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 ||
+                 (family == NUMC_FAMILY_I8 && same_in && in_dt == NUMC_DTYPE_INT8));
   // int8: only when every A row and every column of B is k-contiguous (B
   // stored transposed); other int8 batches run the packed kernel
   for (size_t i = 0; direct && family == NUMC_FAMILY_I8 && i < count; i++)
@@ -489,7 +490,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
 }
 
 // Autotune the batch schedule: time the whole batch (graph replays) under each
-// candidate schedule -- the direct kernel (FP32 and int8 batches) or the
+// candidate schedule -- the direct kernel (FP32, split-BF16 and int8 batches) or the
 // packed one, and the threshold above which a problem takes the larger tile
 // class -- and keep the fastest for the family.  Returns the chosen threshold
 // (numc_matmul_batched_schedule reports the kernel).
@@ -502,7 +503,7 @@ extern "C" NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, Num
   int err = numc_gpu_init(ctx);
   if (err) return err;
   static const int cand[] = {0, 32, 48, 64, 96, 128, 192, 256, 1 << 30};
-  const bool can_direct = family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8;
+  const bool can_direct = family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_I8;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
   HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
@@ -548,7 +549,7 @@ extern "C" NUMC_API int numc_matmul_batched_schedule(int family, int *direct, in
 extern "C" NUMC_API int numc_matmul_batched_set_schedule(int family, int direct, int threshold) {
   if (family < 0 || family >= NUMC_FAMILY_COUNT)
     return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched_set_schedule: bad family %d", family);
-  if (direct && family != NUMC_FAMILY_F32 && family != NUMC_FAMILY_I8)
+  if (direct && family != NUMC_FAMILY_F32 && family != NUMC_FAMILY_SPLIT_BF16 && family != NUMC_FAMILY_I8)
     return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched_set_schedule: no direct batched kernel for family %d", family);
   numc_batch_set_direct(family, direct != 0);
   numc_batch_set_threshold(family, threshold);

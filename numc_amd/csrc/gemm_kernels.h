@@ -226,6 +226,14 @@ __device__ inline int int_products(int s, int maskA, int maskB, int *pa0, int *p
 // for epilogues that take none).  Integer outputs: the low 32 bits, added by
 // the first digit pass only.  Split-K slices store raw partial sums and their
 // reduction adds the bias.
+// Global (address space 1) views of generic pointers.  Operand and output
+// pointers read from a problem table in memory are generic pointers, and
+// generic (flat) accesses count on both the vector-memory and the LDS
+// counters: the compiler then waits for every access in flight
+// (vmcnt(0) lgkmcnt(0)) before using any load, which empties a load pipeline
+// at every step.  Every operand and output here lives in device memory.
+#define NUMC_GLOBAL(T, p) ((__attribute__((address_space(1))) T *)(p))
+
 template <int EPI> struct EpiBias { using T = uint32_t; };
 template <> struct EpiBias<EPI_F32B> { using T = float; };
 template <> struct EpiBias<EPI_F64B> { using T = double; };
@@ -236,17 +244,17 @@ template <int EPI>
 __device__ inline typename EpiBias<EPI>::T epi_bias(const GemmArgs &g, int col) {
   using T = typename EpiBias<EPI>::T;
   if constexpr (EPI == EPI_F32B || EPI == EPI_F64B) {
-    return col < g.N ? ((const T *)g.bias)[(long long)col * g.bias_s] : (T)0;
+    return col < g.N ? NUMC_GLOBAL(const T, g.bias)[(long long)col * g.bias_s] : (T)0;
   } else if constexpr (EPI == EPI_BF16B || EPI == EPI_F16B) {
     if (col >= g.N) return 0.f;
-    const uint32_t b = ((const uint16_t *)g.bias)[(long long)col * g.bias_s];
+    const uint32_t b = NUMC_GLOBAL(const uint16_t, g.bias)[(long long)col * g.bias_s];
     return EPI == EPI_BF16B ? bf16_bits_to_f32(b) : f16_bits_to_f32(b);
   } else if constexpr (EPI == EPI_INTB) {
     if (g.shift_pass || col >= g.N) return 0u;
     const long long b = (long long)col * g.bias_s;
-    return g.out_bytes == 1 ? (uint32_t)((const uint8_t *)g.bias)[b]
-           : g.out_bytes == 2 ? (uint32_t)((const uint16_t *)g.bias)[b]
-                              : ((const uint32_t *)g.bias)[b];
+    return g.out_bytes == 1 ? (uint32_t)NUMC_GLOBAL(const uint8_t, g.bias)[b]
+           : g.out_bytes == 2 ? (uint32_t)NUMC_GLOBAL(const uint16_t, g.bias)[b]
+                              : NUMC_GLOBAL(const uint32_t, g.bias)[b];
   } else {
     return 0u;
   }
@@ -256,30 +264,30 @@ template <int EPI, class Raw>
 __device__ inline void epi_store(const GemmArgs &g, int row, int col, Raw v, typename EpiBias<EPI>::T bias) {
   const long long off = (long long)row * g.rso + (long long)col * g.cso;
   if constexpr (EPI == EPI_F32) {
-    ((float *)g.C)[off] = (float)v;
+    NUMC_GLOBAL(float, g.C)[off] = (float)v;
   } else if constexpr (EPI == EPI_F32B) {
-    ((float *)g.C)[off] = (float)v + bias;
+    NUMC_GLOBAL(float, g.C)[off] = (float)v + bias;
   } else if constexpr (EPI == EPI_F64) {
-    ((double *)g.C)[off] = (double)v;
+    NUMC_GLOBAL(double, g.C)[off] = (double)v;
   } else if constexpr (EPI == EPI_F64B) {
-    ((double *)g.C)[off] = (double)v + bias;
+    NUMC_GLOBAL(double, g.C)[off] = (double)v + bias;
   } else if constexpr (EPI == EPI_BF16 || EPI == EPI_BF16B || EPI == EPI_F16 || EPI == EPI_F16B) {
     float r = (float)v;
     if constexpr (EPI == EPI_BF16B || EPI == EPI_F16B) r += bias;
-    ((uint16_t *)g.C)[off] = (uint16_t)(EPI == EPI_BF16 || EPI == EPI_BF16B ? f32_to_bf16_bits(r) : f32_to_f16_bits(r));
+    NUMC_GLOBAL(uint16_t, g.C)[off] = (uint16_t)(EPI == EPI_BF16 || EPI == EPI_BF16B ? f32_to_bf16_bits(r) : f32_to_f16_bits(r));
   } else {
     // integer digit pass s adds (sum << 8s); the first pass writes the output,
     // with the bias (mod 2^w) under EPI_INTB
     uint32_t x = (uint32_t)(int)v << (8 * g.shift_pass);
     if constexpr (EPI == EPI_INTB) x += bias;
     if (g.out_bytes == 1) {
-      uint8_t *p = (uint8_t *)g.C + off;
+      __attribute__((address_space(1))) uint8_t *p = NUMC_GLOBAL(uint8_t, g.C) + off;
       *p = (uint8_t)(g.shift_pass ? (uint32_t)*p + x : x);
     } else if (g.out_bytes == 2) {
-      uint16_t *p = (uint16_t *)g.C + off;
+      __attribute__((address_space(1))) uint16_t *p = NUMC_GLOBAL(uint16_t, g.C) + off;
       *p = (uint16_t)(g.shift_pass ? (uint32_t)*p + x : x);
     } else {
-      uint32_t *p = (uint32_t *)g.C + off;
+      __attribute__((address_space(1))) uint32_t *p = NUMC_GLOBAL(uint32_t, g.C) + off;
       *p = g.shift_pass ? *p + x : x;
     }
   }

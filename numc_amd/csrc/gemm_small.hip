@@ -31,12 +31,14 @@
 
 namespace {
 
-// a vector load from an address aligned only to its elements (global vector
-// loads on gfx950 take any byte alignment)
-template <class V> __device__ __attribute__((always_inline)) inline V ldu(const void *p) {
-  V v;
-  __builtin_memcpy(&v, p, sizeof(V));
-  return v;
+// a 16-byte vector type with byte alignment (global vector loads on gfx950
+// take any byte alignment)
+typedef unsigned int u32x4_b __attribute__((ext_vector_type(4), aligned(1)));
+
+// a 16-byte load from an address aligned only to its elements
+__device__ __attribute__((always_inline)) inline uint4 ldu16(const unsigned char *p) {
+  const u32x4_b v = *NUMC_GLOBAL(const u32x4_b, p);
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 // What one fragment lane slot holds: E source elements of type S (16 bytes of
@@ -70,23 +72,28 @@ template <> struct Direct<MfF16, MODE_SINGLE> {
 
 // E elements (k0 .. k0+E-1) of one row starting at p (= the row's element k0),
 // k stride ks bytes; zeros at k >= kend.  KC: the k stride is the element
-// size, so a whole slot is one (or two) 16-byte loads.
-template <class S, int E, bool KC>
+// size, so a whole slot is one (or two) 16-byte loads.  FULL: every lane's
+// E elements lie below kend (a whole k-fragment inside K), so the loads take
+// no per-lane conditions (no divergent branches in the main loop).
+template <class S, int E, bool KC, bool FULL = false>
 __device__ __attribute__((always_inline)) inline void load_slot(const unsigned char *p, long long ks, int k0, int kend,
                                                                 S (&v)[E]) {
   constexpr int NQ = E * (int)sizeof(S) / 16;
-  if (KC && k0 + E <= kend) {
+  if constexpr (FULL && !KC) {
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = *NUMC_GLOBAL(const S, p + (long long)e * ks);
+  } else if (KC && (FULL || k0 + E <= kend)) {
     union {
       uint4 q[NQ];
       S e[E];
     } u;
 #pragma unroll
-    for (int q = 0; q < NQ; q++) u.q[q] = ldu<uint4>(p + 16 * q);
+    for (int q = 0; q < NQ; q++) u.q[q] = ldu16(p + 16 * q);
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = u.e[e];
   } else {
 #pragma unroll
-    for (int e = 0; e < E; e++) v[e] = k0 + e < kend ? *(const S *)(p + (long long)e * ks) : (S)0;
+    for (int e = 0; e < E; e++) v[e] = k0 + e < kend ? *NUMC_GLOBAL(const S, p + (long long)e * ks) : (S)0;
   }
 }
 
@@ -94,15 +101,18 @@ __device__ __attribute__((always_inline)) inline void load_slot(const unsigned c
 template <class MF, int MODE, class S, int E>
 __device__ __attribute__((always_inline)) inline void to_frag(const S (&v)[E], u32x4 *out, int stride) {
   if constexpr (MODE == MODE_SPLIT3) {
+    // hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even by
+    // v_cvt_pk_bf16_f32 (two elements per instruction): the packs' split
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
     uint32_t hi[4], lo[4];
 #pragma unroll
     for (int e = 0; e < 8; e += 2) {
-      const float x0 = v[e], x1 = v[e + 1];
-      const uint32_t h0 = f32_to_bf16_bits(x0), h1 = f32_to_bf16_bits(x1);
-      const uint32_t l0 = f32_to_bf16_bits(x0 - __uint_as_float(h0 << 16));
-      const uint32_t l1 = f32_to_bf16_bits(x1 - __uint_as_float(h1 << 16));
-      hi[e / 2] = (h0 & 0xFFFFu) | (h1 << 16);
-      lo[e / 2] = (l0 & 0xFFFFu) | (l1 << 16);
+      const f32x2 x = {v[e], v[e + 1]};
+      const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+      const f32x2 r = {x.x - __uint_as_float(h << 16), x.y - __uint_as_float(h & 0xFFFF0000u)};
+      hi[e / 2] = h;
+      lo[e / 2] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
     }
     out[0] = u32x4{hi[0], hi[1], hi[2], hi[3]};
     out[stride] = u32x4{lo[0], lo[1], lo[2], lo[3]};
@@ -154,7 +164,10 @@ __device__ __attribute__((always_inline)) inline void direct_tile(const GemmArgs
   const int nkf = (g.K + KF - 1) / KF;
   const int wk = WK == 1 ? 0 : w;
   const int f0 = (int)((long long)wk * nkf / WK), f1 = (int)((long long)(wk + 1) * nkf / WK);
-  const int n = f1 - f0;
+  // the ring runs over the whole k-fragments; a last fragment that K cuts
+  // (K % KF != 0) is multiplied after it, with per-lane masked loads
+  const bool part = g.K % KF != 0 && f1 == nkf && f1 > f0;
+  const int n = f1 - f0 - (part ? 1 : 0);
   const int ksub = (lane / T) * E;
   const unsigned char *pa[TM], *pb[TN];
 #pragma unroll
@@ -179,29 +192,22 @@ __device__ __attribute__((always_inline)) inline void direct_tile(const GemmArgs
 
   // register ring of PF fragment steps: [slot][plane][tile]
   u32x4 af[PF][P][TM], bfr[PF][P][TN];
-  auto load = [&](int s, int slot) __attribute__((always_inline)) {
-    if (s < n) {
-      const int k0 = (f0 + s) * KF + ksub;
+  // step s's fragments into ring slot `slot`: FULL, a whole k-fragment inside
+  // K (no per-lane conditions); otherwise the fragment K cuts, zeros past K
+  auto load = [&](int s, int slot, auto full) __attribute__((always_inline)) {
+    constexpr bool FULL = decltype(full)::value;
+    const int k0 = (f0 + s) * KF + ksub;
 #pragma unroll
-      for (int i = 0; i < TM; i++) {
-        S v[E];
-        load_slot<S, E, AK>(pa[i] + s * a_step, g.a_ks, k0, g.K, v);
-        to_frag<MF, MODE>(v, &af[slot][0][i], TM);
-      }
+    for (int i = 0; i < TM; i++) {
+      S v[E];
+      load_slot<S, E, AK, FULL>(pa[i] + s * a_step, g.a_ks, k0, g.K, v);
+      to_frag<MF, MODE>(v, &af[slot][0][i], TM);
+    }
 #pragma unroll
-      for (int j = 0; j < TN; j++) {
-        S v[E];
-        load_slot<S, E, BK>(pb[j] + s * b_step, g.b_ks, k0, g.K, v);
-        to_frag<MF, MODE>(v, &bfr[slot][0][j], TN);
-      }
-    } else {
-#pragma unroll
-      for (int p = 0; p < P; p++) {
-#pragma unroll
-        for (int i = 0; i < TM; i++) af[slot][p][i] = u32x4{0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int j = 0; j < TN; j++) bfr[slot][p][j] = u32x4{0u, 0u, 0u, 0u};
-      }
+    for (int j = 0; j < TN; j++) {
+      S v[E];
+      load_slot<S, E, BK, FULL>(pb[j] + s * b_step, g.b_ks, k0, g.K, v);
+      to_frag<MF, MODE>(v, &bfr[slot][0][j], TN);
     }
   };
   auto mfmas = [&](int slot) __attribute__((always_inline)) {
@@ -220,15 +226,34 @@ __device__ __attribute__((always_inline)) inline void direct_tile(const GemmArgs
       }
   };
 
-  const int npad = (n + PF - 1) / PF * PF;
+  // The ring's loads are unconditional: a load past the last whole fragment
+  // repeats that fragment into a slot no MFMA reads, so the steady-state loop
+  // has no branches and the compiler's waits count exactly the loads issued
+  // after the one awaited.  The loop runs the whole groups of PF steps; the
+  // rest (fewer than PF, already loaded) follow, then the cut fragment.
+  using Full = std::true_type;
+  if (n > 0) {
+    const int nmain = n / PF * PF, rest = n - nmain;
 #pragma unroll
-  for (int s = 0; s < PF - 1; s++) load(s, s);
-  for (int base = 0; base < npad; base += PF) {
+    for (int s = 0; s < PF - 1; s++) load(min(s, n - 1), s, Full());
+    for (int base = 0; base < nmain; base += PF) {
 #pragma unroll
-    for (int u = 0; u < PF; u++) {
-      load(base + u + PF - 1, (u + PF - 1) % PF);
-      mfmas(u);
+      for (int u = 0; u < PF; u++) {
+        // (each step's loads stay in their step: the scheduler would sink
+        // them towards their use PF - 1 steps later)
+        __builtin_amdgcn_sched_barrier(0);
+        load(min(base + u + PF - 1, n - 1), (u + PF - 1) % PF, Full());
+        mfmas(u);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int u = 0; u < PF - 1; u++)
+      if (u < rest) mfmas(u);
+  }
+  if (part) { // the fragment K cuts, in k order after the others
+    load(n, 0, std::false_type());
+    mfmas(0);
   }
 
   const int row0 = bm * TM * T, col0 = bn * TN * T;
@@ -316,25 +341,25 @@ __global__ void __launch_bounds__(WK * 64) k_gemm_direct(const GemmArgs g) {
 // is ordered longest-first.  Each problem's k strides pick whole
 // 16-byte loads or element loads.
 // ---------------------------------------------------------------------------
-template <class MF, int EPI, int TM, int TN, int PF>
+template <class MF, int MODE, int EPI, int TM, int TN, int PF>
 __device__ __attribute__((always_inline)) inline void batch_wave_item(const GemmArgs &g, int bm, int bn, bool ak,
                                                                       bool bk) {
   // int8: k-contiguous operands only (the host sends nothing else; k-strided
   // bytes would take 16 loads per lane slot)
   if constexpr (std::is_same<MF, MfI8>::value) {
-    direct_tile<MF, MODE_SINGLE, TM, TN, 1, PF, EPI, true, true, false>(g, bm, bn, nullptr);
+    direct_tile<MF, MODE, TM, TN, 1, PF, EPI, true, true, false>(g, bm, bn, nullptr);
     return;
   }
-  if (ak && bk) direct_tile<MF, MODE_SINGLE, TM, TN, 1, PF, EPI, true, true, false>(g, bm, bn, nullptr);
-  else if (ak) direct_tile<MF, MODE_SINGLE, TM, TN, 1, PF, EPI, true, false, false>(g, bm, bn, nullptr);
-  else if (bk) direct_tile<MF, MODE_SINGLE, TM, TN, 1, PF, EPI, false, true, false>(g, bm, bn, nullptr);
-  else direct_tile<MF, MODE_SINGLE, TM, TN, 1, PF, EPI, false, false, false>(g, bm, bn, nullptr);
+  if (ak && bk) direct_tile<MF, MODE, TM, TN, 1, PF, EPI, true, true, false>(g, bm, bn, nullptr);
+  else if (ak) direct_tile<MF, MODE, TM, TN, 1, PF, EPI, true, false, false>(g, bm, bn, nullptr);
+  else if (bk) direct_tile<MF, MODE, TM, TN, 1, PF, EPI, false, true, false>(g, bm, bn, nullptr);
+  else direct_tile<MF, MODE, TM, TN, 1, PF, EPI, false, false, false>(g, bm, bn, nullptr);
 }
 
-template <class MF, int EPI, int OCC>
+template <class MF, int MODE, int EPI, int OCC>
 __global__ void __launch_bounds__(256, OCC) k_gemm_batched_direct(const BatchDirect *probs, const int4 *items,
                                                                    int nitems, int *counter) {
-  constexpr long long es = sizeof(typename Direct<MF, MODE_SINGLE>::S);
+  constexpr long long es = sizeof(typename Direct<MF, MODE>::S);
   const int lane = threadIdx.x & 63;
   for (;;) {
     int it = 0;
@@ -362,10 +387,10 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_batched_direct(const BatchDir
     const bool ak = pr.a_ks == es, bk = pr.b_ks == es;
     // class: bit 1 = 64 rows (else 32), bit 0 = 64 columns (else 32)
     switch (w.w) {
-    case 3: batch_wave_item<MF, EPI, 2, 2, 3>(g, w.y, w.z, ak, bk); break;
-    case 2: batch_wave_item<MF, EPI, 2, 1, 3>(g, w.y, w.z, ak, bk); break;
-    case 1: batch_wave_item<MF, EPI, 1, 2, 3>(g, w.y, w.z, ak, bk); break;
-    default: batch_wave_item<MF, EPI, 1, 1, 4>(g, w.y, w.z, ak, bk); break;
+    case 3: batch_wave_item<MF, MODE, EPI, 2, 2, 3>(g, w.y, w.z, ak, bk); break;
+    case 2: batch_wave_item<MF, MODE, EPI, 2, 1, 3>(g, w.y, w.z, ak, bk); break;
+    case 1: batch_wave_item<MF, MODE, EPI, 1, 2, 3>(g, w.y, w.z, ak, bk); break;
+    default: batch_wave_item<MF, MODE, EPI, 1, 1, 4>(g, w.y, w.z, ak, bk); break;
     }
   }
 }
@@ -420,19 +445,26 @@ const GemmConfig *numc_direct_configs(int family, int *count) {
   return t;
 }
 
-// resident blocks per CU of the direct batched kernel (its grid is this many per CU)
-int numc_batch_direct_occupancy(int family) { (void)family; return 2; }
+// resident blocks per CU of the direct batched kernel (its grid is this many
+// per CU; the kernels' launch bounds ask the compiler for that many)
+constexpr int kBatchDirectOcc[NUMC_FAMILY_COUNT] = {2, 2, 1, 1, 2, 1, 1};
+int numc_batch_direct_occupancy(int family) {
+  return family >= 0 && family < NUMC_FAMILY_COUNT ? kBatchDirectOcc[family] : 1;
+}
 
 int numc_batch_direct_launch(int family, const BatchDirect *probs, const int4 *items, int nitems, int *counter,
                              int grid, hipStream_t st, NumcCtx *prof) {
   hipLaunchKernelGGL(k_fill_counter, dim3(1), dim3(64), 0, st, counter);
   if (prof) numc_prof_begin(prof, PROF_BATCHED, family);
   if (family == NUMC_FAMILY_F32)
-    hipLaunchKernelGGL((k_gemm_batched_direct<MfF32, EPI_F32, 2>), dim3(grid), dim3(256), 0, st, probs, items, nitems,
-                       counter);
+    hipLaunchKernelGGL((k_gemm_batched_direct<MfF32, MODE_SINGLE, EPI_F32, kBatchDirectOcc[NUMC_FAMILY_F32]>),
+                       dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  else if (family == NUMC_FAMILY_SPLIT_BF16)
+    hipLaunchKernelGGL((k_gemm_batched_direct<MfBF16, MODE_SPLIT3, EPI_F32, kBatchDirectOcc[NUMC_FAMILY_SPLIT_BF16]>),
+                       dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   else
-    hipLaunchKernelGGL((k_gemm_batched_direct<MfI8, EPI_INT, 2>), dim3(grid), dim3(256), 0, st, probs, items, nitems,
-                       counter);
+    hipLaunchKernelGGL((k_gemm_batched_direct<MfI8, MODE_SINGLE, EPI_INT, kBatchDirectOcc[NUMC_FAMILY_I8]>),
+                       dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   if (prof) numc_prof_end(prof);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "batched direct launch");

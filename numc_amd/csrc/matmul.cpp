@@ -131,8 +131,8 @@ static bool call_tuning_enabled() {
 // tiles when they alone make 256 tiles, else 32x32.  Otherwise a packed
 // configuration by rounds of 256 CUs x per-block work at a tile-dependent
 // efficiency, plus the split-K reduction.
-static const int kDirectLog2K[NUMC_FAMILY_COUNT] = {11, 8, -1, 11, 11, 11, 11};
-static const int kDirectLog2MN[NUMC_FAMILY_COUNT] = {22, 20, -1, 20, 22, 20, 20};
+static const int kDirectLog2K[NUMC_FAMILY_COUNT] = {13, 12, -1, 14, 13, 12, 12};
+static const int kDirectLog2MN[NUMC_FAMILY_COUNT] = {24, 24, -1, 22, 22, 22, 22};
 
 static int heuristic_config(int family, int M, int N, int K, bool allow_direct) {
   if (allow_direct && kDirectLog2K[family] >= 0 && (long long)K <= (1LL << kDirectLog2K[family]) &&

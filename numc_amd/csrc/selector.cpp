@@ -368,9 +368,10 @@ int numc_selector_choose(int family, int M, int N, int K, int flags, bool allow_
 // GPU busy, which is not what a full batch needs.
 static int g_batch_thresh[NUMC_FAMILY_COUNT] = {64, 64, 64, 64, 64, 64, 64};
 
-// FP32 and int8 batches run the direct kernel until numc_matmul_batched_tune
-// measures otherwise; split-BF16 has only the packed one
-static bool g_batch_direct[NUMC_FAMILY_COUNT] = {true, false, false, false, true, false, false};
+// FP32, split-BF16 and int8 batches run the direct kernel until
+// numc_matmul_batched_tune measures otherwise; the other families have only
+// the packed one
+static bool g_batch_direct[NUMC_FAMILY_COUNT] = {true, true, false, false, true, false, false};
 
 bool numc_batch_direct(int family) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -380,7 +381,8 @@ bool numc_batch_direct(int family) {
 void numc_batch_set_direct(int family, bool direct) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (family >= 0 && family < NUMC_FAMILY_COUNT)
-    g_batch_direct[family] = direct && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_I8);
+    g_batch_direct[family] =
+        direct && (family == NUMC_FAMILY_F32 || family == NUMC_FAMILY_SPLIT_BF16 || family == NUMC_FAMILY_I8);
 }
 
 int numc_batch_threshold(int family) {

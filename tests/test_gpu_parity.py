@@ -594,13 +594,17 @@ def _set_schedule(fam, direct, thresh):
 
 @pytest.mark.parametrize("direct", [1, 0], ids=["direct", "packed"])
 @pytest.mark.parametrize("thresh", [0, 64, 1 << 30], ids=["all_large", "t64", "all_small"])
-def test_batched_f32_schedules_views(ctx, direct, thresh):
-    """both batched kernels and both tile classes on uneven problems whose
-    operands are row-major, stored transposed, or start one element past a
-    16-byte boundary; results against the oracle"""
+@pytest.mark.parametrize("fam,method,extra", [(nc.FAMILY_F32, nc.METHOD_PLAIN, 0.0),
+                                              (nc.FAMILY_SPLIT_BF16, nc.METHOD_SPLIT_BF16, 3 * 2.0 ** -16)],
+                         ids=["f32", "split_bf16"])
+def test_batched_f32_schedules_views(ctx, direct, thresh, fam, method, extra):
+    """both batched kernels and every tile class, FP32 and split-BF16, on
+    uneven problems whose operands are row-major, stored transposed, or start
+    one element past a 16-byte boundary; results against the oracle (split:
+    plus the split's representation bound)"""
     rng = np.random.default_rng(19)
-    before = nc.batched_schedule(nc.FAMILY_F32)
-    _set_schedule(nc.FAMILY_F32, direct, thresh)
+    before = nc.batched_schedule(fam)
+    _set_schedule(fam, direct, thresh)
     try:
         As, Bs, Os, ref = [], [], [], []
         for t, (M, N, K) in enumerate(zip(_logu(rng, 1, 400, 24), _logu(rng, 1, 400, 24), _logu(rng, 1, 600, 24))):
@@ -620,11 +624,11 @@ def test_batched_f32_schedules_views(ctx, direct, thresh):
             Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
             ref.append((a, b))
         for rep in range(2):   # capture, then replay
-            nc.check(nc.matmul_batched(ctx, As, Bs, Os), "batched")
+            nc.check(nc.matmul_batched(ctx, As, Bs, Os, method), "batched")
             for (a, b), O in zip(ref, Os):
-                assert cw_err(O.numpy(), a, b) <= (a.shape[1] + 2) * U
+                assert cw_err(O.numpy(), a, b) <= extra + (a.shape[1] + 2) * U
     finally:
-        _set_schedule(nc.FAMILY_F32, *before)
+        _set_schedule(fam, *before)
 
 
 @pytest.mark.parametrize("odt,flags", [(np.int32, nc.OPT_WIDEN), (np.int8, 0)], ids=["widen", "wrap"])
