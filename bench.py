@@ -383,13 +383,11 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
         H = nc.BatchHandles(As, Bs, Os)   # the handle arrays, built once as a C caller keeps them
         nc.check(nc.matmul_batched(ctx, H, method=method, flags=flags), "batched warmup")  # builds the graph
         ctx.synchronize()
-        ctx.mark(2)
-        for _ in range(reps):
-            nc.check(nc.matmul_batched(ctx, H, method=method, flags=flags), "batched")
-        ctx.mark(3)
-        t_batched = ctx.elapsed_ms(2, 3) / reps
+        # whole calls (graph replays) after the clock warm-up, then the same
+        # calls with per-kernel events
         t = time_whole_and_kernels(nc, ctx, lambda: nc.matmul_batched(ctx, H, method=method, flags=flags), fam,
-                                   max(2, steps // 2), 1, batched=True)
+                                   max(reps, steps), 2, batched=True)
+        t_batched = t["ms_per_call"]
         # one call per problem, same stream, no graph
         t_sep = float("nan")
         if separate:
@@ -412,7 +410,7 @@ def bench_batched(nc, count, seed, reps, steps=10, warmup=2, cores=16, with_cpu=
              "pack_launches_per_call": t["pack_launches_per_call"],
              "roofline": roofline_entry(flop, nbytes, t["kernel_ms"], peak, unit,
                                         "persistent batched GEMM kernel, %s operands (avg of %d launches)" %
-                                        ("direct" if direct else "packed", max(2, steps // 2)))}
+                                        ("direct" if direct else "packed", max(reps, steps)))}
         if with_cpu:
             r, why = cpu[0 if dt == nc.INT8 else 8]
             if r:

@@ -28,6 +28,36 @@
 int numc_matmul_one(const NumcArray *a, const NumcArray *b, NumcArray *out, const NumcMatmulOpts *opts,
                     bool finish); // matmul.cpp
 
+// One work queue per XCD: every tile of a problem goes to one XCD's queue, so
+// the tiles that re-read a problem's rows and columns (or packed panels)
+// share that XCD's L2.  Problems are dealt longest-first to the least loaded
+// queue; each queue runs its longest tiles first; a block or wave whose queue
+// is empty takes from the next ones.  work: (predicted tile time, item with
+// the problem in .x), reordered queue by queue; qstart: each queue's first item.
+static void deal_to_queues(std::vector<std::pair<double, int4>> &work, size_t count,
+                           int (&qstart)[NUMC_BATCH_QUEUES + 1]) {
+  std::vector<double> ptime(count, 0.0);
+  for (const auto &x : work) ptime[x.second.x] += x.first;
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ptime[x] > ptime[y]; });
+  std::vector<int> bin(count, 0);
+  double load[NUMC_BATCH_QUEUES] = {0};
+  for (int i : order) {
+    if (ptime[i] == 0.0) continue;
+    const int b = (int)(std::min_element(load, load + NUMC_BATCH_QUEUES) - load);
+    bin[i] = b;
+    load[b] += ptime[i];
+  }
+  std::stable_sort(work.begin(), work.end(), [&](const std::pair<double, int4> &x, const std::pair<double, int4> &y) {
+    const int bx = bin[x.second.x], by = bin[y.second.x];
+    return bx != by ? bx < by : x.first > y.first;
+  });
+  for (int b = 0; b <= NUMC_BATCH_QUEUES; b++) qstart[b] = 0;
+  for (const auto &x : work) qstart[bin[x.second.x] + 1]++;
+  for (int b = 0; b < NUMC_BATCH_QUEUES; b++) qstart[b + 1] += qstart[b];
+}
+
 namespace {
 // the launch a cached graph holds, kept to relaunch its kernels eagerly when
 // the context profiles kernels (events cannot time kernels inside a graph)
@@ -254,11 +284,10 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       for (int x = 0; x < nbm; x++)
         for (int y = 0; y < nbn; y++) work.push_back({tile_time, make_int4((int)i, x, y, cls)});
     }
-    std::stable_sort(work.begin(), work.end(), [](const std::pair<double, int4> &x, const std::pair<double, int4> &y) {
-      return x.first > y.first;
-    });
     if (work.size() > (size_t)INT_MAX)
       return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched: too many tiles for one launch (%zu)", work.size());
+    int qstart[NUMC_BATCH_QUEUES + 1];
+    deal_to_queues(work, count, qstart);
     const int nitems = (int)work.size();
     if (nitems > 0) {
       std::vector<int4> items(work.size());
@@ -266,7 +295,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       const size_t o_prob = 0;
       const size_t o_item = align_up(o_prob + dp.size() * sizeof(BatchDirect), 256);
       const size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
-      const size_t total = o_cnt + 256;
+      const size_t total = o_cnt + NUMC_BATCH_COUNTER_BYTES;
       BatchEntry *e = cache->victim();
       HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
       e->release();
@@ -275,6 +304,9 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       unsigned char *base = (unsigned char *)e->dev;
       HIP_TRY(hipMemcpyAsync(base + o_prob, dp.data(), dp.size() * sizeof(BatchDirect), hipMemcpyHostToDevice, ctx->stream), "H2D");
       HIP_TRY(hipMemcpyAsync(base + o_item, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      // the queue starts follow the counters (which the graph clears)
+      HIP_TRY(hipMemcpyAsync(base + o_cnt + NUMC_BATCH_QSTART * sizeof(int), qstart, sizeof qstart,
+                             hipMemcpyHostToDevice, ctx->stream), "H2D");
       HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
       int cus = 256;
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
@@ -341,7 +373,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       return p.pred / (tiles > 0 ? tiles : 1.0);
     };
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tile_time(x) > tile_time(y); });
-    std::vector<int4> items;
+    std::vector<std::pair<double, int4>> work;
     std::vector<PackArgs> descs;
     std::vector<long long> prefix(1, 0);
     std::vector<BatchProblem> probs(count);
@@ -365,11 +397,15 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       if (!p.C) continue;
       const int nbm = (int)cdivll(p.M, p.bm), nbn = (int)cdivll(p.N, p.bn);
       for (int x = 0; x < nbm; x++)
-        for (int y = 0; y < nbn; y++) items.push_back(make_int4((int)i, x, y, p.cls));
+        for (int y = 0; y < nbn; y++) work.push_back({tile_time(i), make_int4((int)i, x, y, p.cls)});
     }
-    if (items.size() > (size_t)INT_MAX || prefix.back() > (long long)INT_MAX)
+    if (work.size() > (size_t)INT_MAX || prefix.back() > (long long)INT_MAX)
       return NUMC_SET_ERROR(NUMC_ERR_VALUE, "matmul_batched: too many tiles for one launch (%zu work items, %lld pack blocks)",
-                            items.size(), prefix.back());
+                            work.size(), prefix.back());
+    int qstart[NUMC_BATCH_QUEUES + 1];
+    deal_to_queues(work, count, qstart);
+    std::vector<int4> items(work.size());
+    for (size_t x = 0; x < work.size(); x++) items[x] = work[x].second;
     const int ndesc = (int)descs.size();
     const int nitems = (int)items.size();
 
@@ -385,7 +421,7 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
       size_t o_prob = align_up(o_own + owner.size() * sizeof(int), 256);
       size_t o_item = align_up(o_prob + probs.size() * sizeof(BatchProblem), 256);
       size_t o_cnt = align_up(o_item + items.size() * sizeof(int4), 256);
-      size_t o_pack = align_up(o_cnt + 256, 256);
+      size_t o_pack = align_up(o_cnt + NUMC_BATCH_COUNTER_BYTES, 256);
       size_t total = o_pack + (size_t)packed + 256;
       BatchEntry *e = cache->victim();
       HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -422,6 +458,8 @@ extern "C" NUMC_API int numc_matmul_batched(NumcCtx *ctx, size_t count, NumcArra
         HIP_TRY(hipMemcpyAsync(base + o_own, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream), "H2D");
       HIP_TRY(hipMemcpyAsync(base + o_prob, probs.data(), probs.size() * sizeof(BatchProblem), hipMemcpyHostToDevice, ctx->stream), "H2D");
       HIP_TRY(hipMemcpyAsync(base + o_item, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice, ctx->stream), "H2D");
+      HIP_TRY(hipMemcpyAsync(base + o_cnt + NUMC_BATCH_QSTART * sizeof(int), qstart, sizeof qstart,
+                             hipMemcpyHostToDevice, ctx->stream), "H2D");
       HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 
       int cus = 256;

@@ -76,7 +76,16 @@ struct BatchDirect {
   int out_bytes;
 };
 int numc_batch_direct_occupancy(int family);
-// direct batched kernel (FP32 and int8 families; gemm_small.hip): classes 0 = 64x64, 1 = 32x32 tiles
+// The direct batched kernel's work queues, one per XCD: `counter` points at
+// NUMC_BATCH_COUNTER_BYTES of device memory holding queue q's counter at int
+// q * NUMC_BATCH_COUNTER_STRIDE (cleared by the launch) and the queues' item
+// starts (NUMC_BATCH_QUEUES + 1 ints, set by the host) at int NUMC_BATCH_QSTART.
+#define NUMC_BATCH_QUEUES 8
+#define NUMC_BATCH_COUNTER_STRIDE 32
+#define NUMC_BATCH_QSTART (NUMC_BATCH_QUEUES * NUMC_BATCH_COUNTER_STRIDE)
+#define NUMC_BATCH_COUNTER_BYTES ((NUMC_BATCH_QSTART + NUMC_BATCH_QUEUES + 1) * 4 + 60)
+// direct batched kernel (FP32, split-BF16 and int8 families; gemm_small.hip): tile classes of 64 or
+// 32 rows by 64 or 32 columns
 int numc_batch_direct_launch(int family, const BatchDirect *probs, const int4 *items, int nitems, int *counter,
                              int grid, hipStream_t st, NumcCtx *prof);
 

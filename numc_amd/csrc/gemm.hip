@@ -1010,13 +1010,24 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_batched(const BatchProblem *p
   static_assert(OCC * LDSV * 16 <= 160 * 1024, "OCC blocks must fit one CU's LDS");
   __shared__ u32x4 lds[LDSV];
   __shared__ int s_item;
+  // per-XCD work queues (gemm.h): the block takes from its own XCD's queue
+  // (block b runs on XCD b % 8), then from the others in turn; a queue found
+  // empty stays empty, so after finding all of them empty the block is done
+  (void)nitems;
+  const int *qs = counter + NUMC_BATCH_QSTART;
+  int q = blockIdx.x % NUMC_BATCH_QUEUES, left = NUMC_BATCH_QUEUES;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) s_item = atomicAdd(counter + q * NUMC_BATCH_COUNTER_STRIDE, 1);
     __syncthreads(); // also: the previous tile's LDS reads are over
     const int it = s_item;
     __syncthreads(); // everyone read s_item before thread 0 overwrites it
-    if (it >= nitems) return;
-    const int4 w = items[it];
+    const int q0 = qs[q];
+    if (it >= qs[q + 1] - q0) {
+      if (--left == 0) return;
+      q = (q + 1) % NUMC_BATCH_QUEUES;
+      continue;
+    }
+    const int4 w = items[q0 + it];
     const BatchProblem pr = probs[w.x];
     GemmArgs g{};
     g.A = pr.A;
@@ -2255,7 +2266,8 @@ int numc_i64_launch(NumcCtx *ctx, const GemmProblem &p) {
 int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix, const int *owner,
                       long long units, const BatchProblem *probs, const int4 *items, int nitems, int *counter,
                       int grid, hipStream_t st, NumcCtx *prof) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, 1LL, 0u);
+  // the queue counters (the ints before the queue starts)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)counter, (long long)NUMC_BATCH_QSTART, 0u);
   const unsigned nb = (unsigned)units; // blocks (numc_batch_pack_blocks summed)
   if (nb) {
     if (prof) numc_prof_begin(prof, PROF_PACK, family);

@@ -131,6 +131,11 @@ __global__ void k_fill_counter(int *c) {
   if (threadIdx.x == 0) *c = 0;
 }
 
+// the direct batched kernel's queue counters
+__global__ void k_clear_queues(int *c) {
+  if (threadIdx.x < NUMC_BATCH_QUEUES) c[threadIdx.x * NUMC_BATCH_COUNTER_STRIDE] = 0;
+}
+
 template <class Raw> __device__ __attribute__((always_inline)) inline Raw add_raw(Raw a, Raw b) {
   if constexpr (std::is_integral<Raw>::value) return (Raw)((uint32_t)a + (uint32_t)b); // mod 2^32
   else return a + b;
@@ -361,12 +366,25 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_batched_direct(const BatchDir
                                                                    int nitems, int *counter) {
   constexpr long long es = sizeof(typename Direct<MF, MODE>::S);
   const int lane = threadIdx.x & 63;
+  // queue q's items are items[qs[q], qs[q+1]); a wave takes from its own
+  // XCD's queue (blocks are dealt to the 8 XCDs in turn, so block b runs on
+  // XCD b % 8), then from the others in turn; a queue found empty stays empty
+  // (its counter only grows), so after finding all of them empty the wave is
+  // done
+  (void)nitems;
+  const int *qs = counter + NUMC_BATCH_QSTART;
+  int q = blockIdx.x % NUMC_BATCH_QUEUES, left = NUMC_BATCH_QUEUES;
   for (;;) {
     int it = 0;
-    if (lane == 0) it = atomicAdd(counter, 1);
+    if (lane == 0) it = atomicAdd(counter + q * NUMC_BATCH_COUNTER_STRIDE, 1);
     it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-    if (it >= nitems) return;
-    const int4 w = items[it];
+    const int q0 = qs[q];
+    if (it >= qs[q + 1] - q0) {
+      if (--left == 0) return;
+      q = (q + 1) % NUMC_BATCH_QUEUES;
+      continue;
+    }
+    const int4 w = items[q0 + it];
     const BatchDirect pr = probs[w.x];
     GemmArgs g{};
     g.Araw = pr.A;
@@ -454,7 +472,7 @@ int numc_batch_direct_occupancy(int family) {
 
 int numc_batch_direct_launch(int family, const BatchDirect *probs, const int4 *items, int nitems, int *counter,
                              int grid, hipStream_t st, NumcCtx *prof) {
-  hipLaunchKernelGGL(k_fill_counter, dim3(1), dim3(64), 0, st, counter);
+  hipLaunchKernelGGL(k_clear_queues, dim3(1), dim3(64), 0, st, counter);
   if (prof) numc_prof_begin(prof, PROF_BATCHED, family);
   if (family == NUMC_FAMILY_F32)
     hipLaunchKernelGGL((k_gemm_batched_direct<MfF32, MODE_SINGLE, EPI_F32, kBatchDirectOcc[NUMC_FAMILY_F32]>),
