@@ -654,6 +654,35 @@ def test_batched_int8_direct_b_transposed(ctx, odt, flags):
         _set_schedule(nc.FAMILY_I8, *before)
 
 
+def test_split_direct_batch_bits(ctx):
+    """the direct batched kernel splits float32 into bf16 hi/lo in registers
+    (hardware round to nearest even) and adds lo*hi, hi*lo, hi*hi per 16-k
+    fragment in k order, as the packed kernel does after the pack's software
+    split: both give the same bits, ragged K included"""
+    rng = np.random.default_rng(23)
+    before = nc.batched_schedule(nc.FAMILY_SPLIT_BF16)
+    _set_schedule(nc.FAMILY_SPLIT_BF16, 1, 32)
+    try:
+        shapes = [(40, 70, 300), (128, 64, 128), (3, 7, 5), (65, 200, 33), (96, 96, 1000)]
+        As, Bs, Os, Ps = [], [], [], []
+        for t, (M, N, K) in enumerate(shapes):
+            a = rand_f32(rng, (M, K)) * np.float32(2.0 ** (t - 2))
+            b = rand_f32(rng, (K, N))
+            As.append(ctx.from_numpy(a))
+            Bs.append(ctx.from_numpy(b))
+            Os.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+            Ps.append(ctx.array((M, N), nc.FLOAT32, zeros=True))
+        nc.check(nc.matmul_batched(ctx, As, Bs, Os, nc.METHOD_SPLIT_BF16), "batched split")
+        packed = [i for i in range(nc.config_count(nc.FAMILY_SPLIT_BF16))
+                  if not nc.config_get(nc.FAMILY_SPLIT_BF16, i)["direct"]
+                  and nc.config_get(nc.FAMILY_SPLIT_BF16, i)["split_k"] == 1]
+        for A, B, O, P in zip(As, Bs, Os, Ps):
+            nc.check(nc.matmul(A, B, P, nc.METHOD_SPLIT_BF16, packed[0]), "packed split")
+            assert np.array_equal(O.numpy().view(np.uint32), P.numpy().view(np.uint32))
+    finally:
+        _set_schedule(nc.FAMILY_SPLIT_BF16, *before)
+
+
 def test_batched_tune_picks_a_schedule(ctx):
     """the tuner times the direct and packed kernels and keeps one; results
     under the kept schedule stay exact"""
