@@ -432,8 +432,6 @@ __device__ inline void flags_or(int *flags, int mask) {
   if (mask & ~cur) atomicOr(slot, mask);
 }
 
-// one lane slot t of every plane, any strides (the whole wave calls it: the
-// plane mask takes a ballot)
 // The P signed base-256 digits of x (x = sum_p d_p 256^p mod 2^32, each d_p
 // in [-128, 127]) into byte e of the lanes' plane words w[p].  32-bit
 // arithmetic is enough: every product sum is taken mod 2^32, and carries
@@ -448,6 +446,8 @@ __device__ __attribute__((always_inline)) inline void digit_bytes(uint32_t x, ui
   }
 }
 
+// one lane slot t of every plane, any strides (the whole wave calls it: the
+// plane mask takes a ballot)
 template <class S, int P>
 __device__ __attribute__((always_inline)) inline void pack_digits_unit(const PackArgs &a, long long t) {
   bool live = t < (long long)a.nRT * a.nKF * 64;
@@ -1055,12 +1055,12 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_batched(const BatchProblem *p
 static const int kBatchClass[3][2][3] = {
     {{128, 128, 16}, {64, 64, 32}},   // f32
     {{128, 128, 32}, {64, 64, 32}},   // split-BF16
-    {{128, 128, 64}, {64, 64, 64}},   // int8 (one digit plane)
+    {{128, 128, 32}, {64, 64, 64}},   // int8 (one digit plane)
 };
 
 // resident blocks per CU of the batched kernel (its grid is this many per CU)
 int numc_batch_occupancy(int family) {
-  return family == NUMC_FAMILY_I8 ? 4 : family == NUMC_FAMILY_F32 ? 3 : 2;
+  return family == NUMC_FAMILY_I8 ? 3 : family == NUMC_FAMILY_F32 ? 3 : 2;
 }
 
 int numc_batch_class_shape(int family, int cls, int *bm, int *bn, int *kc) {
@@ -2281,8 +2281,11 @@ int numc_batch_launch(int family, const PackArgs *descs, const long long *prefix
     hipLaunchKernelGGL((k_gemm_batched<MfF32, MODE_SINGLE, EPI_F32, 128, 128, 16, 3, 64, 64, 32, 3, 3>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   else if (family == NUMC_FAMILY_SPLIT_BF16)
     hipLaunchKernelGGL((k_gemm_batched<MfBF16, MODE_SPLIT3, EPI_F32, 128, 128, 32, 2, 64, 64, 32, 4, 2>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+  // int8: the 128 x 128 class in k-chunks of 32 with 6 stages, 3 blocks per CU.  Its
+  // tiles average 4 k-chunks of 64; the deeper pipeline took the 4096-problem batch's
+  // GEMM from 0.63 to 0.58 ms (k-chunks of 64 with 2 stages at 4 blocks per CU before)
   else
-    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 64, 2, 64, 64, 64, 4, 4>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
+    hipLaunchKernelGGL((k_gemm_batched<MfI8, MODE_SINGLE, EPI_INT, 128, 128, 32, 6, 64, 64, 64, 4, 3>), dim3(grid), dim3(256), 0, st, probs, items, nitems, counter);
   if (prof) numc_prof_end(prof);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return numc_hip_fail(e, "batched launch");
