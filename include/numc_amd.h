@@ -103,10 +103,11 @@ NUMC_API int numc_amd_gemm_noalloc(void *stream, int dtype, int out_dtype, size_
 /* Autotune the batched schedule for this batch: times the whole batch under
  * each candidate schedule and keeps the fastest for the batch's dtype family
  * in later numc_matmul_batched calls.  A schedule is the kernel -- direct
- * (float32 and int8 inputs: operands read where they lie, no pack pass;
- * 64x64 / 32x32 tiles whose waves split K) or packed (one pack launch, then
- * 128x128 / 64x64 tiles) -- and the threshold above which a problem's M and N
- * give it the larger tile.  Returns the threshold. */
+ * (float32 plain or split-BF16, and int8 inputs: operands read where they
+ * lie, no pack pass; each wave takes one 64x64, 64x32, 32x64 or 32x32 tile
+ * over the whole K) or packed (one pack launch, then 128x128 / 64x64 tiles)
+ * -- and the threshold above which a problem's M and N give it the larger
+ * tile.  Returns the threshold. */
 NUMC_API int numc_matmul_batched_tune(NumcCtx *ctx, size_t count, NumcArray *const *a,
                                       NumcArray *const *b, NumcArray *const *out,
                                       const NumcMatmulOpts *opts, int reps);
