@@ -57,30 +57,72 @@ __global__ void k_zero_strided(unsigned char *c, int M, int N, long long rso, lo
 // so the destination offset of plane 0 is t*16.
 // ===========================================================================
 
-template <class S> __device__ inline S ld(const void *p, long long i) { return ((const S *)p)[i]; }
+template <class S> __device__ inline S ld(const void *p, long long i) { return NUMC_GLOBAL(const S, p)[i]; }
 
 // a vector load from an address aligned only to its elements: global vector
 // loads on gfx950 take any byte alignment (a chunk that straddles two cache
 // lines costs one more line), and the compiler emits the same
 // global_load_dwordx4 for this copy as for an aligned load
+typedef unsigned int u32x4_any __attribute__((ext_vector_type(4), aligned(1)));
 template <class V> __device__ __attribute__((always_inline)) inline V ldu(const void *p) {
-  V v;
-  __builtin_memcpy(&v, p, sizeof(V));
-  return v;
+  static_assert(sizeof(V) == 16, "16-byte chunks");
+  const u32x4_any v = *NUMC_GLOBAL(const u32x4_any, p);
+  V r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
 }
 
-// 16 bytes (16 / sizeof(S) values) of a chunk of an edge tile, zero past
-// n_ok values: one vector load when the chunk is whole, else element loads (volatile keeps the compiler from merging them with the
-// interior tiles' vector loads)
+// One 16-byte chunk of an edge tile (16 / sizeof(S) values from src, n_ok
+// of them inside the matrix, zero past them).  A whole chunk is one vector
+// load; a partial one is read as the 16-byte-aligned chunks that hold its
+// first and its last valid byte (an aligned chunk never crosses a page, so
+// both lie in pages the matrix occupies); a chunk with no valid byte loads
+// nothing.  A thread issues the loads of all its chunks first (edge_issue),
+// then shifts each into place and clears the bytes past its valid ones
+// (edge_finish): element loads would each wait for the one before.
+typedef unsigned int u32x4_al __attribute__((ext_vector_type(4)));
+struct EdgeChunk {
+  u32x4_al lo, hi;
+  int s, nb; // byte offset of the chunk in lo:hi, valid bytes
+};
+
 template <class S>
-__device__ __attribute__((always_inline)) inline uint4 load_chunk_edge(const S *src, long long n_ok) {
+__device__ __attribute__((always_inline)) inline EdgeChunk edge_issue(const S *src, long long n_ok) {
   constexpr int EPC = 16 / sizeof(S);
-  if (n_ok >= EPC) return ldu<uint4>(src); // a whole chunk
-  union { uint4 q; S e[EPC]; } u;
-  const volatile S *vs = src;
-#pragma unroll
-  for (int e = 0; e < EPC; e++) u.e[e] = e < n_ok ? vs[e] : (S)0;
-  return u.q;
+  const int nb = n_ok <= 0 ? 0 : n_ok >= EPC ? 16 : (int)n_ok * (int)sizeof(S);
+  const unsigned long long p = (unsigned long long)src;
+  EdgeChunk e;
+  e.nb = nb;
+  e.s = nb == 16 ? 0 : (int)(p & 15);
+  e.lo = u32x4_al{0u, 0u, 0u, 0u};
+  e.hi = u32x4_al{0u, 0u, 0u, 0u};
+  if (nb == 16) {
+    e.lo = *NUMC_GLOBAL(const u32x4_any, src);
+  } else if (nb > 0) {
+    const unsigned long long c0 = p - (unsigned long long)e.s;
+    e.lo = *NUMC_GLOBAL(const u32x4_al, c0);
+    if (e.s + nb > 16) e.hi = *NUMC_GLOBAL(const u32x4_al, c0 + 16);
+  }
+  return e;
+}
+
+__device__ __attribute__((always_inline)) inline uint4 edge_finish(const EdgeChunk &e) {
+  typedef unsigned long long u64;
+  // lo:hi as four 64-bit words; the chunk is bytes s .. s+15 of them, taken
+  // with 64-bit funnel shifts of whole words (written out rather than as an
+  // array indexed by s: the compiler turns such a select into a scratch
+  // store and a load at a variable offset)
+  const u64 L0 = ((u64)e.lo.y << 32) | e.lo.x, L1 = ((u64)e.lo.w << 32) | e.lo.z;
+  const u64 H0 = ((u64)e.hi.y << 32) | e.hi.x, H1 = ((u64)e.hi.w << 32) | e.hi.z;
+  const bool up = (e.s & 8) != 0;
+  const u64 a = up ? L1 : L0, b = up ? H0 : L1, c = up ? H1 : H0;
+  const int sh = (e.s & 7) * 8;
+  u64 o0 = sh ? (a >> sh) | (b << (64 - sh)) : a;
+  u64 o1 = sh ? (b >> sh) | (c << (64 - sh)) : b;
+  // clear the bytes past the valid ones
+  if (e.nb < 8) o0 &= (1ull << (8 * e.nb)) - 1ull;
+  o1 = e.nb <= 8 ? 0ull : e.nb < 16 ? o1 & ((1ull << (8 * (e.nb - 8))) - 1ull) : o1;
+  return make_uint4((uint32_t)o0, (uint32_t)(o0 >> 32), (uint32_t)o1, (uint32_t)(o1 >> 32));
 }
 
 
@@ -245,17 +287,17 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_k(const Pack
       v[it] = ldu<float4>(base + (idx >> 5) * a.rs + (idx & 31) * 4);
     }
   } else {
+    EdgeChunk e[4];
 #pragma unroll
     for (int it = 0; it < 4; it++) {
-      int idx = tid + it * 256;
-      int r = idx >> 5, c4 = (idx & 31) * 4;
-      long long row = row0 + r;
-      v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-      const int k = k0 + c4;
-      if (row < a.R && k < a.K) {
-        const uint4 q = load_chunk_edge<float>((const float *)a.src + row * a.rs + k, (long long)a.K - k);
-        v[it] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w));
-      }
+      const int idx = tid + it * 256, k = k0 + (idx & 31) * 4;
+      const long long row = row0 + (idx >> 5);
+      e[it] = edge_issue<float>((const float *)a.src + row * a.rs + k, row < a.R ? (long long)a.K - k : 0);
+    }
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const uint4 q = edge_finish(e[it]);
+      v[it] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w));
     }
   }
 #pragma unroll
@@ -333,19 +375,17 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
       v[it] = ldu<float4>(base + (long long)(idx >> 5) * a.ks + (idx & 31) * 4);
     }
   } else {
+    EdgeChunk e[4];
 #pragma unroll
     for (int it = 0; it < 4; it++) {
-      int idx = tid + it * 256;
-      int kk = idx >> 5, r4 = (idx & 31) * 4;
-      int k = k0 + kk;
-      v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (k < a.K) {
-        // (volatile keeps the compiler from merging these edge loads with
-        // the interior tile's vector loads)
-        const float *src = (const float *)a.src + (long long)k * a.ks + row0 + r4;
-        const uint4 q = load_chunk_edge<float>(src, (long long)a.R - (row0 + r4));
-        v[it] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w));
-      }
+      const int idx = tid + it * 256, k = k0 + (idx >> 5), r4 = (idx & 31) * 4;
+      e[it] = edge_issue<float>((const float *)a.src + (long long)k * a.ks + row0 + r4,
+                                k < a.K ? (long long)a.R - (row0 + r4) : 0);
+    }
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+      const uint4 q = edge_finish(e[it]);
+      v[it] = make_float4(__uint_as_float(q.x), __uint_as_float(q.y), __uint_as_float(q.z), __uint_as_float(q.w));
     }
   }
 #pragma unroll
@@ -1341,12 +1381,14 @@ __device__ __attribute__((always_inline)) inline void digits_tile_k(const PackAr
       q[it] = ldu<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * EPC);
     }
   } else {
+    EdgeChunk e[NL];
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256, r = idx / CPR, c = (idx % CPR) * EPC;
-      q[it] = make_uint4(0u, 0u, 0u, 0u);
-      if (row0 + r < a.R && k0 + c < a.K) q[it] = load_chunk_edge<S>(base + r * a.rs + c, (long long)a.K - (k0 + c));
+      e[it] = edge_issue<S>(base + r * a.rs + c, row0 + r < a.R ? (long long)a.K - (k0 + c) : 0);
     }
+#pragma unroll
+    for (int it = 0; it < NL; it++) q[it] = edge_finish(e[it]);
   }
 #pragma unroll
   for (int it = 0; it < NL; it++) {
@@ -1399,13 +1441,14 @@ __device__ __attribute__((always_inline)) inline void digits_tile_rows(const Pac
       q[it] = ldu<uint4>(base + (long long)(idx / CPK) * a.ks + (idx % CPK) * EPC);
     }
   } else {
+    EdgeChunk e[NL];
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256, kk = idx / CPK, r = (idx % CPK) * EPC;
-      q[it] = make_uint4(0u, 0u, 0u, 0u);
-      if (k0 + kk < a.K && row0 + r < a.R)
-        q[it] = load_chunk_edge<S>(base + (long long)kk * a.ks + r, (long long)a.R - (row0 + r));
+      e[it] = edge_issue<S>(base + (long long)kk * a.ks + r, k0 + kk < a.K ? (long long)a.R - (row0 + r) : 0);
     }
+#pragma unroll
+    for (int it = 0; it < NL; it++) q[it] = edge_finish(e[it]);
   }
 #pragma unroll
   for (int it = 0; it < NL; it++) {
@@ -1477,11 +1520,6 @@ template <class S> static void launch_pair_digits(int planes, PackArgs &pa, Pack
 // the byte itself, so a fragment lane is 16 source bytes, copied as is
 // ---------------------------------------------------------------------------
 
-// 16 source bytes at p of an edge tile, n_ok of them inside the matrix (zero past it)
-__device__ __attribute__((always_inline)) inline uint4 load16_bytes(const unsigned char *p, long long n_ok) {
-  return load_chunk_edge<unsigned char>(p, n_ok);
-}
-
 // k-contiguous rows: a 32-row x 512-k byte tile (16 fragments).  Each row
 // segment is read as 512 contiguous bytes, staged in LDS (row pitch 528 B,
 // so the 16-byte lane reads of consecutive rows fall on different banks),
@@ -1505,13 +1543,15 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArg
       v[it] = ldu<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * 16);
     }
   } else {
+    EdgeChunk e[NL];
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256, r = idx / CPR, c = idx % CPR;
       const int k = kb * KB + c * 16;
-      v[it] = make_uint4(0u, 0u, 0u, 0u);
-      if (rt * 32 + r < a.R && k < a.K) v[it] = load16_bytes(base + r * a.rs + c * 16, (long long)a.K - k);
+      e[it] = edge_issue<unsigned char>(base + r * a.rs + c * 16, rt * 32 + r < a.R ? (long long)a.K - k : 0);
     }
+#pragma unroll
+    for (int it = 0; it < NL; it++) v[it] = edge_finish(e[it]);
   }
 #pragma unroll
   for (int it = 0; it < NL; it++) {
@@ -1567,12 +1607,14 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_rows(const Pack
 #pragma unroll
     for (int q = 0; q < 4; q++) v[q] = ldu<uint4>(base + (long long)(kq * 4 + q) * a.ks + rg * 16);
   } else {
+    EdgeChunk e[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int k = kk + q;
-      v[q] = make_uint4(0u, 0u, 0u, 0u);
-      if (k < a.K && r < a.R) v[q] = load16_bytes((const unsigned char *)a.src + (long long)k * a.ks + r, a.R - r);
+      e[q] = edge_issue<unsigned char>((const unsigned char *)a.src + (long long)k * a.ks + r, k < a.K ? a.R - r : 0);
     }
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = edge_finish(e[q]);
   }
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -1659,13 +1701,15 @@ __device__ __attribute__((always_inline)) inline void raw16_tile_rows(const Pack
 #pragma unroll
     for (int q = 0; q < 4; q++) v[q] = ldu<uint4>(base + (long long)(kq * 4 + q) * a.ks + rg * 8);
   } else {
+    EdgeChunk e[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int k = k0 + kq * 4 + q;
       const long long r = row0 + rg * 8;
-      v[q] = make_uint4(0u, 0u, 0u, 0u);
-      if (k < a.K && r < a.R) v[q] = load_chunk_edge<uint16_t>(src + (long long)k * a.ks + r, a.R - r);
+      e[q] = edge_issue<uint16_t>(src + (long long)k * a.ks + r, k < a.K ? a.R - r : 0);
     }
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = edge_finish(e[q]);
   }
 #pragma unroll
   for (int j = 0; j < 8; j++) { // row rg*8 + j: the j-th 16-bit value of each load
