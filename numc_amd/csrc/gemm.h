@@ -53,6 +53,7 @@ struct PackArgs {
   int mode;           // batched pack: 0 k-contiguous tiles, 1 row-contiguous tiles, 2 lane slots,
                       // 3 int8 byte tiles (k-contiguous), 4 int8 byte tiles (row-contiguous)
   long long src_zs, dst_zs; // strided batch: bytes between slices of the source and of the packed planes
+  int tile_rt;        // int8 / 16-bit k-contiguous byte tiles: row tiles per block (1, 2 or 4)
 };
 
 // one problem of the persistent batched kernel (packed operands in a workspace)

@@ -1520,22 +1520,26 @@ template <class S> static void launch_pair_digits(int planes, PackArgs &pa, Pack
 // the byte itself, so a fragment lane is 16 source bytes, copied as is
 // ---------------------------------------------------------------------------
 
-// k-contiguous rows: a 32-row x 512-k byte tile (16 fragments).  Each row
-// segment is read as 512 contiguous bytes, staged in LDS (row pitch 528 B,
-// so the 16-byte lane reads of consecutive rows fall on different banks),
-// then written out as 16 whole fragments.
-__device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArgs &a, long long bid, uint32_t *tile) {
-  constexpr int KB = 512, CPR = KB / 16, PW = CPR + 1, NF = KB / 32;
-  static_assert(32 * PW * 4 <= PACK_TILE_WORDS, "tile");
-  constexpr int NL = 32 * CPR / 256; // 16-byte loads per thread
+// k-contiguous rows: a tile of 32*RT rows x 512/RT bytes of k (16
+// fragments), RT = 1, 2 or 4 as a.tile_rt says (set on the host with the
+// block count, so that short rows fill a block with more of them).  Each row
+// segment is read as 512/RT contiguous bytes, staged in LDS (row pitch 16
+// bytes more, so the 16-byte lane reads of consecutive rows fall on
+// different banks), then written out as 16 whole fragments.
+template <int RT>
+__device__ __attribute__((always_inline)) inline void bytes_tile_k_rt(const PackArgs &a, long long bid, uint32_t *tile) {
+  constexpr int KB = 512 / RT, CPR = KB / 16, PW = CPR + 1, NF = KB / 32, ROWS = 32 * RT;
+  static_assert(ROWS * PW * 4 <= PACK_TILE_WORDS, "tile");
+  constexpr int NL = ROWS * CPR / 256; // 16-byte loads per thread
   uint4 *t = (uint4 *)tile;
   const int nkb = (a.nKF + NF - 1) / NF;
-  const long long rt = bid / nkb;
+  const long long rb = bid / nkb; // the block's RT row tiles start at row tile rb * RT
   const int kb = (int)(bid % nkb);
   const int tid = threadIdx.x;
-  const unsigned char *base = (const unsigned char *)a.src + rt * 32 * a.rs + (long long)kb * KB;
+  const long long row0 = rb * ROWS;
+  const unsigned char *base = (const unsigned char *)a.src + row0 * a.rs + (long long)kb * KB;
   uint4 v[NL];
-  if (rt * 32 + 32 <= a.R && kb * KB + KB <= a.K) {
+  if (row0 + ROWS <= a.R && kb * KB + KB <= a.K) {
     // interior tile: every load in flight at once
 #pragma unroll
     for (int it = 0; it < NL; it++) {
@@ -1548,7 +1552,7 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArg
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256, r = idx / CPR, c = idx % CPR;
       const int k = kb * KB + c * 16;
-      e[it] = edge_issue<unsigned char>(base + r * a.rs + c * 16, rt * 32 + r < a.R ? (long long)a.K - k : 0);
+      e[it] = edge_issue<unsigned char>(base + r * a.rs + c * 16, row0 + r < a.R ? (long long)a.K - k : 0);
     }
 #pragma unroll
     for (int it = 0; it < NL; it++) v[it] = edge_finish(e[it]);
@@ -1561,16 +1565,24 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArg
   __syncthreads();
   bool any = false;
 #pragma unroll
-  for (int it = 0; it < NF * 64 / 256; it++) {
+  for (int it = 0; it < 16 * 64 / 256; it++) {
     const int u = tid + it * 256, f = u >> 6, lane = u & 63;
-    const int kf = kb * NF + f;
-    if (kf < a.nKF) {
-      const uint4 v = t[(lane & 31) * PW + f * 2 + (lane >> 5)];
+    const int j = f / NF, fk = f % NF; // row tile j of the block, its k fragment fk
+    const long long rt = rb * RT + j;
+    const int kf = kb * NF + fk;
+    if (rt < a.nRT && kf < a.nKF) {
+      const uint4 v = t[(j * 32 + (lane & 31)) * PW + fk * 2 + (lane >> 5)];
       *(uint4 *)(a.dst + (rt * a.nKF + kf) * 1024 + lane * 16) = v;
       any |= (v.x | v.y | v.z | v.w) != 0;
     }
   }
   if (__ballot(any) && (threadIdx.x & 63) == 0 && a.flags) flags_or(a.flags, 1);
+}
+
+__device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArgs &a, long long bid, uint32_t *tile) {
+  if (a.tile_rt == 4) bytes_tile_k_rt<4>(a, bid, tile);
+  else if (a.tile_rt == 2) bytes_tile_k_rt<2>(a, bid, tile);
+  else bytes_tile_k_rt<1>(a, bid, tile);
 }
 
 __global__ void __launch_bounds__(256) k_pack_bytes_k(PackArgs a) {
@@ -1840,12 +1852,27 @@ __global__ void __launch_bounds__(256) k_pack_pair(const PackArgs a, const PackA
   else pack_walk<FAM>(b, blockIdx.x - na, tile);
 }
 
+// blocks of bytes_tile_k for an operand, with its row tiles per block
+// (pk.tile_rt): the fewest blocks, so that short rows fill a block with more
+// of them
+static long long bytes_k_blocks(PackArgs &pk) {
+  long long best = -1;
+  for (int rt = 1; rt <= 4; rt *= 2) {
+    const long long n = cdiv((long long)pk.nRT, rt) * cdiv((long long)pk.nKF, 16 / rt);
+    if (best < 0 || n < best) {
+      best = n;
+      pk.tile_rt = rt;
+    }
+  }
+  return best;
+}
+
 // blocks the batched pack gives one operand, and the walk it takes
 int numc_batch_pack_blocks(int family, PackArgs &pk) {
   if (family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16) { // 5/6/7: the raw 16-bit walks
     if (pk.ks == 1) {
       pk.mode = 5;
-      return (int)(pk.nRT * cdiv(pk.nKF, 16));
+      return (int)bytes_k_blocks(pk);
     }
     if (pk.rs == 1) {
       pk.mode = 6;
@@ -1857,7 +1884,7 @@ int numc_batch_pack_blocks(int family, PackArgs &pk) {
   const int kf = family == NUMC_FAMILY_F32 ? 8 : family == NUMC_FAMILY_SPLIT_BF16 ? 16 : 32;
   if (family == NUMC_FAMILY_I8 && pk.ks == 1) { // int8 operands, one plane: byte tiles
     pk.mode = 3;
-    return (int)(pk.nRT * cdiv(pk.nKF, 16));
+    return (int)bytes_k_blocks(pk);
   }
   if (family == NUMC_FAMILY_I8 && pk.rs == 1) {
     pk.mode = 4;
@@ -1878,7 +1905,8 @@ int numc_batch_pack_blocks(int family, PackArgs &pk) {
 template <class S, int P> static void launch_digits_p(PackArgs &pk, unsigned nb, unsigned nz, hipStream_t st) {
   if constexpr (sizeof(S) == 1 && P == 1) {
     if (pk.ks == 1) {
-      hipLaunchKernelGGL(k_pack_bytes_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16)), 1, nz), dim3(256), 0, st, pk);
+      const unsigned nbk = (unsigned)bytes_k_blocks(pk);
+      hipLaunchKernelGGL(k_pack_bytes_k, dim3(nbk, 1, nz), dim3(256), 0, st, pk);
       return;
     }
     if (pk.rs == 1) {
@@ -1929,8 +1957,10 @@ static void launch_pack(int family, NumcDType in_dt, int planes, PackArgs &pk, u
     return;
   }
   if (family == NUMC_FAMILY_BF16 || family == NUMC_FAMILY_F16) { // raw 16-bit copies
-    if (pk.ks == 1)
-      hipLaunchKernelGGL(k_pack_raw16_k, dim3((unsigned)(pk.nRT * cdiv(pk.nKF, 16)), 1, nz), dim3(256), 0, st, pk);
+    if (pk.ks == 1) {
+      const unsigned nbk = (unsigned)bytes_k_blocks(pk);
+      hipLaunchKernelGGL(k_pack_raw16_k, dim3(nbk, 1, nz), dim3(256), 0, st, pk);
+    }
     else if (pk.rs == 1)
       hipLaunchKernelGGL(k_pack_raw16_rows, dim3((unsigned)(cdiv(pk.nRT, 4) * cdiv(pk.nKF, 4)), 1, nz), dim3(256), 0, st, pk);
     else
