@@ -1053,20 +1053,28 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_batched(const BatchProblem *p
   // per-XCD work queues (gemm.h): the block takes from its own XCD's queue
   // (block b runs on XCD b % 8), then from the others in turn; a queue found
   // empty stays empty, so after finding all of them empty the block is done
+  // Thread 0 takes the next item's index while the current tile runs, so the
+  // atomic's round trip is off the path from one tile to the next.
   (void)nitems;
   const int *qs = counter + NUMC_BATCH_QSTART;
   int q = blockIdx.x % NUMC_BATCH_QUEUES, left = NUMC_BATCH_QUEUES;
+  int q0 = qs[q], qn = qs[q + 1] - q0;
+  int nxt = 0;
+  if (threadIdx.x == 0) nxt = atomicAdd(counter + q * NUMC_BATCH_COUNTER_STRIDE, 1);
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter + q * NUMC_BATCH_COUNTER_STRIDE, 1);
+    if (threadIdx.x == 0) s_item = nxt;
     __syncthreads(); // also: the previous tile's LDS reads are over
     const int it = s_item;
     __syncthreads(); // everyone read s_item before thread 0 overwrites it
-    const int q0 = qs[q];
-    if (it >= qs[q + 1] - q0) {
+    if (it >= qn) {
       if (--left == 0) return;
       q = (q + 1) % NUMC_BATCH_QUEUES;
+      q0 = qs[q];
+      qn = qs[q + 1] - q0;
+      if (threadIdx.x == 0) nxt = atomicAdd(counter + q * NUMC_BATCH_COUNTER_STRIDE, 1);
       continue;
     }
+    if (threadIdx.x == 0) nxt = atomicAdd(counter + q * NUMC_BATCH_COUNTER_STRIDE, 1);
     const int4 w = items[q0 + it];
     const BatchProblem pr = probs[w.x];
     GemmArgs g{};
