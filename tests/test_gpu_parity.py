@@ -366,6 +366,50 @@ def test_pack_interior_and_edge_tiles(ctx, dt, offset):
             assert cw_err(got, a, b) <= bound, (method, offset)
 
 
+EDGE_CASES = [(np.int8, nc.FAMILY_I8, nc.METHOD_AUTO), (np.uint8, nc.FAMILY_I8, nc.METHOD_AUTO),
+              (np.int16, nc.FAMILY_I8, nc.METHOD_AUTO), (np.int32, nc.FAMILY_I8, nc.METHOD_AUTO),
+              (np.float32, nc.FAMILY_F32, nc.METHOD_PLAIN),
+              (np.float32, nc.FAMILY_SPLIT_BF16, nc.METHOD_SPLIT_BF16),
+              (np.float16, nc.FAMILY_F16, nc.METHOD_AUTO)]
+
+
+@pytest.mark.parametrize("dt,family,method", EDGE_CASES,
+                         ids=["int8", "uint8", "int16", "int32", "f32", "split_bf16", "f16"])
+def test_pack_partial_chunks_every_offset(ctx, dt, family, method):
+    """The packs' partial 16-byte chunks (a row's last, cut chunk, read as the
+    aligned chunks around it and shifted into place): rows starting at every
+    byte offset within 16, with row lengths that leave every cut, in packed
+    configurations (A k-contiguous, B row-contiguous), the last row ending at
+    the end of its buffer."""
+    cfg = next(i for i in range(nc.config_count(family)) if not nc.config_get(family, i)["direct"])
+    rng = np.random.default_rng(404)
+    es = np.dtype(dt).itemsize
+    M, N = 33, 37
+    for off in range(16 // es):
+        for K in (1, 7, 13, 16, 17, 35):
+            if np.issubdtype(dt, np.integer):
+                info = np.iinfo(dt)
+                base_a = rng.integers(info.min, info.max, (M, K + off), dtype=dt, endpoint=True)
+                base_b = rng.integers(info.min, info.max, (K, N + off), dtype=dt, endpoint=True)
+            else:
+                base_a = rng.standard_normal((M, K + off)).astype(dt)
+                base_b = rng.standard_normal((K, N + off)).astype(dt)
+            A = ctx.from_numpy(base_a).slice(1, off, off + K)
+            B = ctx.from_numpy(base_b).slice(1, off, off + N)
+            a, b = np.ascontiguousarray(base_a[:, off:]), np.ascontiguousarray(base_b[:, off:])
+            O = ctx.array((M, N), nc.DTYPE_OF_NP[np.dtype(dt)], zeros=True)
+            nc.check(nc.matmul(A, B, O, method, cfg), "matmul partial chunks")
+            got = O.numpy()
+            if np.issubdtype(dt, np.integer):
+                assert np.array_equal(got, oracle.matmul_int(a, b)), (off, K)
+            else:
+                bound = (K + 2) * U + (3 * 2.0 ** -16 if method == nc.METHOD_SPLIT_BF16 else 0.0)
+                if dt == np.float16:  # float16 values are exact in float32; the store rounds to 11 bits
+                    bound += 2.0 ** -11
+                    got, a, b = got.astype(np.float32), a.astype(np.float32), b.astype(np.float32)
+                assert cw_err(got, a, b) <= bound, (off, K)
+
+
 def test_pointer_entry_host_and_device(ctx):
     """numc_amd_gemm (pointer-level entry for a host library keeping its own
     arrays) on host pointers with strided views, widening, and device pointers."""
