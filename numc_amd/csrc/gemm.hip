@@ -354,7 +354,10 @@ __global__ void __launch_bounds__(256) k_pack_f32_tiled(PackArgs a) {
 template <int MODE_PK>
 __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const PackArgs &a, long long bid, float *tile) {
   constexpr int TR = 128, TK = 32, PITCH = TK + 1;
-  static_assert(TR * PITCH <= PACK_TILE_WORDS, "tile");
+  // row r of the tile sits at LDS row r + r / 32: the transposing writes (a
+  // wave's lanes at rows 4l + j, one k) and the fragment reads (32
+  // consecutive rows, one k) then both fall on 32 different banks
+  static_assert((TR + TR / 32) * PITCH <= PACK_TILE_WORDS, "tile");
   constexpr int KF = MODE_PK == 0 ? 8 : MODE_PK == 1 ? 16 : 32;
   // consecutive blocks take neighbouring row blocks at one k range, so the
   // segments they read complete each other's source rows
@@ -391,11 +394,11 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
 #pragma unroll
   for (int it = 0; it < 4; it++) {
     const int idx = tid + it * 256;
-    const int kk = idx >> 5, r4 = (idx & 31) * 4;
-    tile[(r4 + 0) * PITCH + kk] = v[it].x;
-    tile[(r4 + 1) * PITCH + kk] = v[it].y;
-    tile[(r4 + 2) * PITCH + kk] = v[it].z;
-    tile[(r4 + 3) * PITCH + kk] = v[it].w;
+    const int kk = idx >> 5, r4 = (idx & 31) * 4, lr = r4 + (r4 >> 5);
+    tile[(lr + 0) * PITCH + kk] = v[it].x;
+    tile[(lr + 1) * PITCH + kk] = v[it].y;
+    tile[(lr + 2) * PITCH + kk] = v[it].z;
+    tile[(lr + 3) * PITCH + kk] = v[it].w;
   }
   __syncthreads();
   constexpr int E = KF / 2;
@@ -408,7 +411,7 @@ __device__ __attribute__((always_inline)) inline void pack_f32_tile_rows(const P
     const int kf = kt * FK + fk;
     if (rt >= a.nRT || kf >= a.nKF) continue;
     const int r = rl * 32 + (lane & 31);
-    const float *t = tile + r * PITCH + fk * KF + (lane >> 5) * E;
+    const float *t = tile + (r + rl) * PITCH + fk * KF + (lane >> 5) * E;
     const long long off = (rt * a.nKF + kf) * 1024 + lane * 16;
     if constexpr (MODE_PK == 0) {
       *(uint4 *)(a.dst + off) = make_uint4(__float_as_uint(t[0]), __float_as_uint(t[1]),
