@@ -75,9 +75,12 @@ template <class V> __device__ __attribute__((always_inline)) inline V ldu(const 
 // the same load marked non-temporal (streamed past the caches): the float
 // packs read each source byte once, and their sources then no longer push
 // the packed planes the GEMM reads next out of the L2 and the MALL (FP8
-// 8192^2 pack 0.135 -> 0.105 ms; split-BF16 0.177 -> 0.162 ms).  The byte
+// 8192^2 pack 0.135 -> 0.105 ms; split-BF16 0.177 -> 0.162 ms; the 16-bit
+// float packs, 8192^2 bf16 0.099 -> 0.087 ms).  The int8 and integer digit
 // packs keep cached loads: their tiles of short rows share lines (int8
-// batch pack 0.25 -> 0.28 ms with streamed loads).
+// batch pack 0.25 -> 0.28 ms with streamed loads), and the int32 digit pack
+// is one round of blocks, where a streamed load's longer trip shows (0.0148
+// -> 0.0164 ms).
 template <class V> __device__ __attribute__((always_inline)) inline V ldu_stream(const void *p) {
   static_assert(sizeof(V) == 16, "16-byte chunks");
   const u32x4_any v = __builtin_nontemporal_load(NUMC_GLOBAL(const u32x4_any, p));
@@ -1561,7 +1564,7 @@ template <class S> static void launch_pair_digits(int planes, PackArgs &pa, Pack
 // segment is read as 512/RT contiguous bytes, staged in LDS (row pitch 16
 // bytes more, so the 16-byte lane reads of consecutive rows fall on
 // different banks), then written out as 16 whole fragments.
-template <int RT>
+template <int RT, bool STREAM>
 __device__ __attribute__((always_inline)) inline void bytes_tile_k_rt(const PackArgs &a, long long bid, uint32_t *tile) {
   constexpr int KB = 512 / RT, CPR = KB / 16, PW = CPR + 1, NF = KB / 32, ROWS = 32 * RT;
   static_assert(ROWS * PW * 4 <= PACK_TILE_WORDS, "tile");
@@ -1580,7 +1583,8 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k_rt(const Pack
 #pragma unroll
     for (int it = 0; it < NL; it++) {
       const int idx = tid + it * 256;
-      v[it] = ldu<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * 16);
+      v[it] = STREAM ? ldu_stream<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * 16)
+                     : ldu<uint4>(base + (idx / CPR) * a.rs + (idx % CPR) * 16);
     }
   } else {
     EdgeChunk e[NL];
@@ -1615,10 +1619,11 @@ __device__ __attribute__((always_inline)) inline void bytes_tile_k_rt(const Pack
   if (__ballot(any) && (threadIdx.x & 63) == 0 && a.flags) flags_or_seen(a.flags, 1, seen);
 }
 
+template <bool STREAM = false>
 __device__ __attribute__((always_inline)) inline void bytes_tile_k(const PackArgs &a, long long bid, uint32_t *tile) {
-  if (a.tile_rt == 4) bytes_tile_k_rt<4>(a, bid, tile);
-  else if (a.tile_rt == 2) bytes_tile_k_rt<2>(a, bid, tile);
-  else bytes_tile_k_rt<1>(a, bid, tile);
+  if (a.tile_rt == 4) bytes_tile_k_rt<4, STREAM>(a, bid, tile);
+  else if (a.tile_rt == 2) bytes_tile_k_rt<2, STREAM>(a, bid, tile);
+  else bytes_tile_k_rt<1, STREAM>(a, bid, tile);
 }
 
 __global__ void __launch_bounds__(256) k_pack_bytes_k(PackArgs a) {
@@ -1719,7 +1724,7 @@ __global__ void __launch_bounds__(256) k_pack_raw16_k(PackArgs a) {
   __shared__ uint32_t tile[PACK_TILE_WORDS];
   a.K *= 2;
   a.rs *= 2;
-  bytes_tile_k(a, blockIdx.x, tile);
+  bytes_tile_k<true>(a, blockIdx.x, tile);
 }
 
 __device__ __attribute__((always_inline)) inline uint32_t u4_at(const uint4 &v, int i) {
@@ -1748,7 +1753,7 @@ __device__ __attribute__((always_inline)) inline void raw16_tile_rows(const Pack
   if (row0 + TR <= a.R && k0 + TK <= a.K) {
     // interior tile: all four loads in flight at once
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = ldu<uint4>(base + (long long)(kq * 4 + q) * a.ks + rg * 8);
+    for (int q = 0; q < 4; q++) v[q] = ldu_stream<uint4>(base + (long long)(kq * 4 + q) * a.ks + rg * 8);
   } else {
     EdgeChunk e[4];
 #pragma unroll
@@ -1833,7 +1838,7 @@ __device__ __attribute__((always_inline)) inline void pack_walk(const PackArgs &
       PackArgs b = a; // the k-contiguous rows as bytes
       b.K *= 2;
       b.rs *= 2;
-      bytes_tile_k(b, u, tile);
+      bytes_tile_k<true>(b, u, tile);
     } else if (a.mode == 6) {
       raw16_tile_rows(a, u, tile);
     } else {
