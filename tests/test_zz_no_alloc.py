@@ -40,6 +40,7 @@ CASES = [  # (name, input dtype, output dtype, numpy in, numpy out, method, flag
     ("int32", nc.INT32, nc.INT32, np.int32, np.int32, nc.METHOD_AUTO, 0, nc.FAMILY_I8),
     ("int8_widen", nc.INT8, nc.INT32, np.int8, np.int32, nc.METHOD_AUTO, nc.OPT_WIDEN, nc.FAMILY_I8),
     ("bf16", nc.BFLOAT16, nc.BFLOAT16, np.uint16, np.uint16, nc.METHOD_AUTO, 0, nc.FAMILY_BF16),
+    ("f16", nc.FLOAT16, nc.FLOAT16, np.float16, np.float16, nc.METHOD_AUTO, 0, nc.FAMILY_F16),
 ]
 
 
@@ -83,6 +84,9 @@ def test_noalloc_matches_numc_amd_gemm(ctx, case):
                 av, bv = nc.bf16_value(a).astype(np.float64), nc.bf16_value(b).astype(np.float64)
                 ref, den = av @ bv, np.abs(av) @ np.abs(bv)
                 assert np.all(np.abs(nc.bf16_value(got) - ref) <= (K + 3) * U * den + 2.0 ** -8 * np.abs(ref))
+            elif dt == nc.FLOAT16:  # float16 values are exact in float32; the store rounds to 11 bits
+                ref, absr = oracle.fref(a.astype(np.float32), b.astype(np.float32))
+                assert oracle.err_metrics(got.astype(np.float32), ref, absr)[0] <= (K + 2) * U + 2.0 ** -11
             elif method == nc.METHOD_FP8_SCALED:
                 qa, qb = oracle.fp8_quantized_inputs(a, b)
                 ref, absr = oracle.fref(qa, qb)
